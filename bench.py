@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Throughput of the LDDRK6 time-advance hot path on one MI355X.
+
+Workload (default): 2-D Navier-Stokes, 4096 x 4096 uniform grid, uniform mean
+flow M=0.5, Gaussian acoustic pulse + isentropic vortex, 13-point filter
+sigma=0.2 in every stage, 40-point buffer zones, FP64.  A "step" is one full
+LDDRK6 step (6 stages of RHS + update + buffer + x/y filter + edges).
+
+Prints one JSON line: Mpts*steps/s over the timed steps, the roofline of the
+dominant kernel (stage kernel, HBM bound) from HIP events on the library's
+stream, the whole-step roofline against the 704 B/pt/step compulsory
+traffic, and the CPU oracle timed on the host as a reported baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpts·steps/s + % HBM roofline; FP64 vs FP32-pert at ≤1e-4 p' error"
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes_per_point_step(nk: int, elem: int) -> float:
+    # 2N-storage LDDRK6: U read 6x + written 6x, W read 5x (stages 2-6) and
+    # written 5x (stages 1-5) -> 22 field accesses per point per step
+    return 22.0 * nk * elem
+
+
+def make_case(args):
+    from navierstokes_amd import config
+    if args.workload == "pulse":
+        case = config.acoustic_pulse_case(args.size or 201)
+        desc = f"euler-acoustic-pulse {case.imax}x{case.jmax} fp64 lddrk6 sigma=0.2 D=30"
+    else:
+        case = config.pulse_vortex_case(args.size or 4096)
+        desc = (f"ns-pulse-vortex {case.imax}x{case.jmax} M=0.5 Re=1e4(pulse) fp64 lddrk6 "
+                f"sigma=0.2 D=40")
+    return case, desc
+
+
+def cpu_baseline(case, steps: int):
+    """The oracle (C restatement of the reference, OpenMP over host cores)
+    timed on a bounded sample of the same workload."""
+    from oracle.oracle import Oracle
+    o = Oracle(case, parallel=True)
+    t0 = time.perf_counter()
+    o.run(case.U0, 0, steps)
+    dt = time.perf_counter() - t0
+    return {"value": case.npoints * steps / dt / 1e6, "unit": "Mpts·steps/s",
+            "cores": int(o.threads()), "kind": "port",
+            "sample": f"{steps} LDDRK6 step(s) of the same {case.imax}x{case.jmax} workload, "
+                      f"OpenMP build of oracle/ns_oracle.c ({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=["ns", "pulse"], default="ns")
+    ap.add_argument("--size", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from navierstokes_amd.solver import Solver
+
+    case, desc = make_case(args)
+    print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
+    s = Solver(case)
+    s.run(0, args.warmup)
+    s.synchronize()
+    print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    s.run(args.warmup, args.steps)
+    s.synchronize()
+    t = time.perf_counter() - t0
+    s.check_finite()
+    ms_step = t / args.steps * 1e3
+    value = case.npoints * args.steps / t / 1e6
+
+    # per-kernel device time (HIP events on the library's stream)
+    s.enable_timing(True)
+    s.run(args.warmup + args.steps, args.timing_steps)
+    kt = {name: s.kernel_time(name)[0] for name in ("rhs", "filter_x", "filter_y", "bc")}
+    s.enable_timing(False)
+    s.close()
+
+    nk, elem = case.nk, 8
+    B_step = alg_bytes_per_point_step(nk, elem)
+    B_stage = B_step / 6.0  # per point per stage-kernel launch (average over the 6 stages)
+    t_stage = kt["rhs"] * 1e-3
+    ach = B_stage * case.npoints / t_stage / 1e9
+    step_ach = B_step * case.npoints / (ms_step * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpts·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "grid": [case.imax, case.jmax], "nk": nk, "precision": "fp64",
+                   "viscous": not case.inviscid, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "stage_kernel (RHS+LDDRK update+buffer)",
+                     "alg_bytes_per_launch": B_stage * case.npoints,
+                     "avg_launch_ms": round(kt["rhs"], 4)},
+        "step_roofline": {"alg_bytes_per_point_step": B_step, "achieved_GBs": round(step_ach, 1),
+                          "frac": round(step_ach / HBM_PEAK_GBS, 4)},
+        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+    }
+    if not args.no_cpu_baseline:
+        print(f"[bench] cpu baseline ({args.cpu_steps} steps)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+/*
+ * ns_api.h -- C ABI of the MI355X (gfx950) time-advance library for the
+ * mathe1s/NavierStokes 2-D solver.
+ *
+ * The reference exposes its hot path as Fortran module procedures that share
+ * state through `module global` (global.f90).  This header is the drop-in
+ * boundary that replaces them: a context (`ns_ctx`) holds, on one GPU, what
+ * `global` holds on the host (scalars, 1-D profiles, base flow, state) and
+ * every reference routine on the path has an entry point here.  The Fortran
+ * shim (navierstokes_amd/fortran/ns_shim.f90) keeps the reference's module
+ * and procedure names and calls these through ISO_C_BINDING; the Python layer
+ * (navierstokes_amd/solver.py) calls them through ctypes.
+ *
+ * Conventions
+ *   - every function returns 0 on success or a negative NS_ERR_* code; the
+ *     message of the last failure on the calling thread is ns_last_error();
+ *   - host arrays use the reference's layout U(1:nk,1:imax,1:jmax):
+ *     element (k,i,j), 0-based, at U[k + nk*(i + imax*j)] (AoS); they are
+ *     borrowed for the duration of the call only;
+ *   - device buffers belong to the context; the state stays resident in HBM
+ *     between calls and is kept as structure-of-arrays planes;
+ *   - one context = one device + one HIP stream; contexts are not thread
+ *     safe, several contexts may coexist.
+ */
+#ifndef NS_API_H
+#define NS_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NS_API_VERSION 1
+
+/* status codes */
+#define NS_OK 0
+#define NS_ERR_PARAM (-1)     /* invalid argument / parameter set */
+#define NS_ERR_HIP (-2)       /* HIP runtime failure */
+#define NS_ERR_NOMEM (-3)     /* device allocation failed */
+#define NS_ERR_STATE (-4)     /* call out of order (e.g. step before profiles) */
+#define NS_ERR_NONFINITE (-5) /* NaN/Inf found by ns_check_finite */
+
+/* precision modes */
+#define NS_PREC_FP64 0      /* reference semantics, FP64 everywhere */
+#define NS_PREC_FP32_PERT 1 /* FP32 perturbation about the FP64 base flow */
+
+/* flags */
+#define NS_FLAG_INVISCID 1u           /* drop the 1/Re, 1/(Re Pr), 1/Pe terms */
+#define NS_FLAG_METRIC_EVERYWHERE 2u  /* apply meshx/meshy at every point (stretched
+                                         grids); default = reference region tests */
+
+typedef struct ns_params {
+    int32_t imax, jmax; /* grid points (global.f90:12) */
+    int32_t nk;         /* 4 = (rho,u,v,p); 5 adds the passive scalar psi */
+    int32_t D;          /* buffer-zone width in points (global.f90:86) */
+    double dm, dn, dt;  /* grid steps and time step (global.f90:11) */
+    double sigmaf;      /* selective-filter strength (global.f90:92) */
+    double gamma, Re, Pr, Pe, alpha_cond, Tref;
+    /* transport fits ln mu = A ln T + B/T + C/T^2 + D and reference values,
+       coefficients.f90:71-123; ns_default_params fills the N2 set */
+    double A_mi, B_mi, C_mi, D_mi, MI0;
+    double A_k, B_k, C_k, D_k, K0;
+    /* acoustic source, rk.f90:171-193 (amplitude 0 disables it) */
+    double amplitude, wf, r0_acous, m_pt_acous, n_pt_acous;
+    int32_t precision; /* NS_PREC_* */
+    uint32_t flags;    /* NS_FLAG_* */
+    int32_t device;    /* HIP device ordinal */
+    int32_t reserved;
+} ns_params;
+
+typedef struct ns_ctx ns_ctx;
+
+/* library / parameters --------------------------------------------------- */
+int32_t ns_api_version(void);
+const char *ns_last_error(void);
+/* dados.in defaults + N2 transport (coefficients.f90:71-123, global.f90:38-39) */
+int32_t ns_default_params(ns_params *p);
+
+/* lifetime: allocates the device state (global.f90 allocatables) */
+int32_t ns_create(const ns_params *p, ns_ctx **out);
+int32_t ns_destroy(ns_ctx *ctx);
+/* run on an external HIP stream (hipStream_t passed as void*; NULL = own) */
+int32_t ns_set_stream(ns_ctx *ctx, void *hip_stream);
+int32_t ns_synchronize(ns_ctx *ctx);
+
+/* inputs produced by init.f90:56-206 / 350-404 on the host ---------------- */
+/* meshx(imax), meshy(jmax), sigmax(imax), sigmay(jmax), m(imax), n(jmax),
+   rhob(jmax), psib(jmax) (psib may be NULL when nk == 4) */
+int32_t ns_set_profiles(ns_ctx *ctx, const double *meshx, const double *meshy,
+                        const double *sigmax, const double *sigmay, const double *m,
+                        const double *n, const double *rhob, const double *psib);
+/* base flow Ub(nk,imax,jmax), init.f90:392-404 */
+int32_t ns_set_base(ns_ctx *ctx, const double *Ub);
+
+/* state ------------------------------------------------------------------- */
+int32_t ns_put_state(ns_ctx *ctx, const double *U);  /* host AoS -> device */
+int32_t ns_get_state(ns_ctx *ctx, double *U);        /* device -> host AoS */
+/* gather npts points (1-based (i,j) pairs) x nk values: euler.f90:156-161 probes */
+int32_t ns_probe(ns_ctx *ctx, int32_t npts, const int32_t *ij, double *out);
+/* NS_ERR_NONFINITE if any state value is NaN/Inf */
+int32_t ns_check_finite(ns_ctx *ctx);
+
+/* time advance ------------------------------------------------------------- */
+/* one LDDRK6 step, rk.f90:12-75 (rk_euler(U,t,iter,nk)) */
+int32_t ns_rk_euler(ns_ctx *ctx, int32_t iter);
+/* nsteps steps with iter = iter0 .. iter0+nsteps-1 (euler.f90:123-130);
+   the step is captured once into a HIP graph and replayed */
+int32_t ns_rk_steps(ns_ctx *ctx, int32_t iter0, int32_t nsteps);
+
+/* operator-level mirrors (host AoS arrays in/out; for parity checks) -------- */
+/* diff.f90:6-38 deropm (+ metricsX :315-350), any nk in 1..8 */
+int32_t ns_deropm(ns_ctx *ctx, int32_t nk, double *dU, const double *U);
+/* diff.f90:127-151 deropn (+ metricsY :276-311) */
+int32_t ns_deropn(ns_ctx *ctx, int32_t nk, double *dU, const double *U);
+/* coefficients.f90:27-59 MOLECULAR_MI, k_coefficient on T(imax,jmax) */
+int32_t ns_transport(ns_ctx *ctx, const double *T, double *MI, double *K1);
+/* rhs.f90:10-46 rhs_euler(dUdt,U,nk) with the source of (ct, iter) */
+int32_t ns_rhs(ns_ctx *ctx, double *dUdt, const double *U, double ct, int32_t iter);
+/* rk.f90:153-169 rk_steep(alphat,betat,ct,U,dudt,w,nk) */
+int32_t ns_rk_steep(ns_ctx *ctx, double alphat, double betat, double *U, const double *dudt,
+                    double *w);
+/* rk.f90:77-111 bufferxy(U,nk) */
+int32_t ns_bufferxy(ns_ctx *ctx, double *U);
+/* filtering.f90:9-96 lddfilterx(U,nk), :98-190 lddfiltery(U,nk) */
+int32_t ns_filter_x(ns_ctx *ctx, double *U);
+int32_t ns_filter_y(ns_ctx *ctx, double *U);
+/* cc.f90:6-68 ccpml(U,nk) */
+int32_t ns_ccpml(ns_ctx *ctx, double *U);
+/* rk.f90:171-193 acousticfont(ct,iter): s_acou(imax,jmax) */
+int32_t ns_acousticfont(ns_ctx *ctx, double ct, int32_t iter, double *s_acou);
+
+/* measurement ----------------------------------------------------------------- */
+/* average device time (ms) of each kernel class over the last ns_rk_steps
+   call with timing enabled: names "rhs", "filter_x", "filter_y", "bc" */
+int32_t ns_enable_timing(ns_ctx *ctx, int32_t on);
+int32_t ns_kernel_time(ns_ctx *ctx, const char *name, double *avg_ms, int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,773 @@
+// ns_api.hip -- C ABI (include/ns_api.h) over the gfx950 kernels.
+//
+// A context owns what the reference keeps in `module global` (global.f90):
+// the scalar parameter block, the 1-D profiles built by init.f90, the base
+// flow Ub, and the state U, all resident on one GPU.  The state lives in three
+// SoA buffers S0..S2 that rotate through each LDDRK stage
+//   S0 --stage--> S1 --filter_x--> S2 --filter_y+edges--> S0
+// so one captured step graph replays without pointer changes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ns_api.h"
+#include "ns_kernels.h"
+
+namespace nsamd {
+hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, hipStream_t s);
+hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s);
+hipError_t launch_edge_i(const EdgeArgs &E, hipStream_t s);
+hipError_t launch_deriv(const double *const *in, double *const *out, int nk, int imax, int jmax, int pitch, bool ydir,
+                        const double *met, double inv_h, hipStream_t s);
+hipError_t launch_transport(const double *T, double *MI, double *K1, size_t n, const Transport<double> &tc,
+                            hipStream_t s);
+hipError_t launch_rk_steep(double *const *U, const double *const *dudt, double *const *W, int nk, int imax, int jmax,
+                           int pitch, double alpha, double beta, double dt, hipStream_t s);
+hipError_t launch_buffer(double *const *U, const double *const *base, int nk, int imax, int jmax, int pitch, int D,
+                         const double *sigx, const double *sigy, hipStream_t s);
+hipError_t launch_source(double *s_out, int imax, int jmax, double amp, const double *gx, const double *gy,
+                         hipStream_t s);
+hipError_t launch_probe(const double *const *U, int nk, int pitch, int npts, const int *ij, double *out,
+                        hipStream_t s);
+hipError_t launch_finite(const double *const *U, int nk, int imax, int jmax, int pitch, int *flag, hipStream_t s);
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
+}  // namespace nsamd
+
+using namespace nsamd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(NS_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+enum TimerClass { TC_STAGE = 0, TC_FX, TC_FY, TC_BC, TC_N };
+const char *TC_NAMES[TC_N] = {"rhs", "filter_x", "filter_y", "bc"};
+
+}  // namespace
+
+struct ns_ctx {
+    ns_params P{};
+    int pitch = 0;
+    size_t plane = 0;  // elements per plane
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double *S[3] = {nullptr, nullptr, nullptr};
+    double *Wr = nullptr;     // LDDRK register
+    double *base = nullptr;   // Ub planes
+    double *work = nullptr;   // operator-API scratch (2 x 8 planes)
+    double *prof = nullptr;   // 1-D profiles
+    double *metx = nullptr, *mety = nullptr, *sigx = nullptr, *sigy = nullptr;
+    double *rhob = nullptr, *psib = nullptr, *gx = nullptr, *gy = nullptr;
+    double **dptrs = nullptr;  // device pointer tables for operator kernels
+    int *dflag = nullptr;
+    bool have_profiles = false, have_base = false, have_state = false;
+    int filt_L = 256, filt_W = 40;
+    hipGraphExec_t graph = nullptr;
+    bool timing = false;
+    hipEvent_t ev[2 * 24] = {};
+    double tsum[TC_N] = {};
+    long tcount[TC_N] = {};
+};
+
+namespace {
+
+double *plane_ptr(double *buf, const ns_ctx *c, int k) { return buf + (size_t)k * c->plane; }
+
+// 1-D profile block: three arrays along i, five along j, one allocation
+struct ProfileLayout {
+    size_t metx, sigx, gx, mety, sigy, rhob, psib, gy, total;
+};
+
+ProfileLayout profile_layout(const ns_params &P)
+{
+    ProfileLayout L;
+    const size_t I = (size_t)P.imax, J = (size_t)P.jmax;
+    L.metx = 0;
+    L.sigx = I;
+    L.gx = 2 * I;
+    L.mety = 3 * I;
+    L.sigy = L.mety + J;
+    L.rhob = L.sigy + J;
+    L.psib = L.rhob + J;
+    L.gy = L.psib + J;
+    L.total = L.gy + J;
+    return L;
+}
+
+size_t profile_count(const ns_params &P) { return profile_layout(P).total; }
+
+Transport<double> make_transport(const ns_params &P)
+{
+    Transport<double> t;
+    t.A_mi = P.A_mi;
+    t.Bt_mi = P.B_mi / P.Tref;
+    t.Ct_mi = P.C_mi / P.Tref / P.Tref;
+    t.D_mi = P.D_mi;
+    t.inv_mi0 = 1.0 / P.MI0;
+    t.A_k = P.A_k;
+    t.Bt_k = P.B_k / P.Tref;
+    t.Ct_k = P.C_k / P.Tref / P.Tref;
+    t.D_k = P.D_k;
+    t.inv_k0 = 1.0 / P.K0;
+    t.Tref = P.Tref;
+    return t;
+}
+
+// Warm-up length of the look-back filter sweep: the homogeneous response of
+// u_i + sigma*sum_{s=1..6} d_{-s} u_{i-s} = f_i must fall below 1e-19.
+int filter_warmup(double sigma)
+{
+    double c[6];
+    for (int s = 1; s <= 6; ++s) c[s - 1] = sigma * filt_d(6 - s);
+    std::vector<double> h(400, 0.0);
+    h[0] = 1.0;
+    for (int n = 1; n < 400; ++n) {
+        double acc = 0.0;
+        for (int s = 1; s <= 6 && s <= n; ++s) acc -= c[s - 1] * h[n - s];
+        h[n] = acc;
+    }
+    for (int W = 16; W < 380; ++W) {
+        double tail = 0.0;
+        for (int n = W; n < 400; ++n) tail += std::fabs(h[n]);
+        if (tail < 1e-19) return W;
+    }
+    return -1;
+}
+
+StageArgs<double> stage_args(ns_ctx *c, int irk, double *Uin, double *Uout, double amp)
+{
+    const ns_params &P = c->P;
+    StageArgs<double> A{};
+    for (int k = 0; k < NKMAX; ++k) {
+        const bool on = k < P.nk;
+        A.U[k] = on ? plane_ptr(Uin, c, k) : nullptr;
+        A.Uout[k] = on ? plane_ptr(Uout, c, k) : nullptr;
+        A.W[k] = on ? plane_ptr(c->Wr, c, k) : nullptr;
+        A.base[k] = on ? plane_ptr(c->base, c, k) : nullptr;
+        A.dudt[k] = nullptr;
+    }
+    A.metx = c->metx;
+    A.mety = c->mety;
+    A.sigx = c->sigx;
+    A.sigy = c->sigy;
+    A.gx = c->gx;
+    A.gy = c->gy;
+    A.imax = P.imax;
+    A.jmax = P.jmax;
+    A.pitch = c->pitch;
+    A.D = P.D;
+    A.inv_dm = 1.0 / P.dm;
+    A.inv_dn = 1.0 / P.dn;
+    A.dt = P.dt;
+    A.alpha = RK_ALPHA[irk];
+    A.beta = RK_BETA[irk];
+    A.gamma = P.gamma;
+    A.inv_Re = 1.0 / P.Re;
+    A.gm1_Re = (P.gamma - 1.0) / P.Re;
+    A.inv_RePr = 1.0 / P.Re / P.Pr;
+    A.inv_Pe = 1.0 / P.Pe;
+    A.alpha_cond = P.alpha_cond;
+    A.amp_src = amp;
+    A.tc = make_transport(P);
+    A.first = irk == 0;
+    A.last = irk == 5;
+    return A;
+}
+
+FilterArgs<double> filter_args(ns_ctx *c, bool xdir, double *in, double *out, bool edge_bc)
+{
+    const ns_params &P = c->P;
+    FilterArgs<double> F{};
+    for (int k = 0; k < NKMAX; ++k) {
+        F.in[k] = k < P.nk ? plane_ptr(in, c, k) : nullptr;
+        F.out[k] = k < P.nk ? plane_ptr(out, c, k) : nullptr;
+    }
+    F.n = xdir ? P.imax : P.jmax;
+    F.m = xdir ? P.jmax : P.imax;
+    F.pitch = c->pitch;
+    F.L = c->filt_L;
+    F.W = c->filt_W;
+    F.sigma = P.sigmaf;
+    F.gamma = P.gamma;
+    F.nk = P.nk;
+    F.edge_bc = edge_bc;
+    return F;
+}
+
+EdgeArgs edge_args(ns_ctx *c, double *U)
+{
+    EdgeArgs E{};
+    for (int k = 0; k < NKMAX; ++k) E.U[k] = k < c->P.nk ? plane_ptr(U, c, k) : nullptr;
+    E.rhob = c->rhob;
+    E.psib = c->P.nk >= 5 ? c->psib : nullptr;
+    E.imax = c->P.imax;
+    E.jmax = c->P.jmax;
+    E.pitch = c->pitch;
+    E.nk = c->P.nk;
+    E.gamma = c->P.gamma;
+    return E;
+}
+
+double stage_amp(const ns_params &P, int irk, int iter)
+{
+    if (P.amplitude == 0.0) return 0.0;
+    return P.amplitude * std::sin(P.wf * (((double)iter + RK_C[irk]) * P.dt));
+}
+
+// one LDDRK6 step on the device, rk.f90:52-73
+int enqueue_step(ns_ctx *c, int iter, bool timed)
+{
+    const ns_params &P = c->P;
+    const bool visc = !(P.flags & NS_FLAG_INVISCID);
+    for (int irk = 0; irk < 6; ++irk) {
+        auto tick = [&](int slot) {
+            if (timed) hipEventRecord(c->ev[slot], c->stream);
+        };
+        StageArgs<double> A = stage_args(c, irk, c->S[0], c->S[1], stage_amp(P, irk, iter));
+        tick(0);
+        HIPCHK(launch_stage(A, P.nk, visc, false, c->stream));
+        tick(1);
+        HIPCHK(launch_filter(filter_args(c, true, c->S[1], c->S[2], false), true, c->stream));
+        tick(2);
+        HIPCHK(launch_filter(filter_args(c, false, c->S[2], c->S[0], true), false, c->stream));
+        tick(3);
+        HIPCHK(launch_edge_i(edge_args(c, c->S[0]), c->stream));
+        tick(4);
+        if (timed) {
+            HIPCHK(hipEventSynchronize(c->ev[4]));
+            for (int t = 0; t < TC_N; ++t) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, c->ev[t], c->ev[t + 1]));
+                c->tsum[t] += ms;
+                c->tcount[t] += 1;
+            }
+        }
+    }
+    return NS_OK;
+}
+
+int check_ready(ns_ctx *c)
+{
+    if (!c) return fail(NS_ERR_PARAM, "null context");
+    if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
+    if (!c->have_base) return fail(NS_ERR_STATE, "ns_set_base has not been called");
+    return NS_OK;
+}
+
+// host AoS (k fastest) <-> device SoA planes
+int upload_aos(ns_ctx *c, double *dst, const double *U, int nk)
+{
+    const ns_params &P = c->P;
+    std::vector<double> tmp((size_t)nk * c->plane, 0.0);
+    for (int j = 0; j < P.jmax; ++j)
+        for (int i = 0; i < P.imax; ++i)
+            for (int k = 0; k < nk; ++k)
+                tmp[(size_t)k * c->plane + (size_t)j * c->pitch + i] = U[k + (size_t)nk * (i + (size_t)P.imax * j)];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int download_aos(ns_ctx *c, const double *src, double *U, int nk)
+{
+    const ns_params &P = c->P;
+    std::vector<double> tmp((size_t)nk * c->plane);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < P.jmax; ++j)
+        for (int i = 0; i < P.imax; ++i)
+            for (int k = 0; k < nk; ++k)
+                U[k + (size_t)nk * (i + (size_t)P.imax * j)] = tmp[(size_t)k * c->plane + (size_t)j * c->pitch + i];
+    return NS_OK;
+}
+
+int set_ptr_table(ns_ctx *c, int slot, double *buf, int nk)
+{
+    double *h[8];
+    for (int k = 0; k < 8; ++k) h[k] = k < nk ? plane_ptr(buf, c, k) : nullptr;
+    HIPCHK(hipMemcpyAsync(c->dptrs + 8 * slot, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    return NS_OK;
+}
+
+void release(ns_ctx *c)
+{
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    for (double *p : {c->S[0], c->S[1], c->S[2], c->Wr, c->base, c->work, c->prof})
+        if (p) hipFree(p);
+    if (c->dptrs) hipFree(c->dptrs);
+    if (c->dflag) hipFree(c->dflag);
+    for (auto &e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ns_api_version(void) { return NS_API_VERSION; }
+
+const char *ns_last_error(void) { return g_err.c_str(); }
+
+int32_t ns_default_params(ns_params *p)
+{
+    if (!p) return fail(NS_ERR_PARAM, "null params");
+    std::memset(p, 0, sizeof *p);
+    // dados.in:1-85 defaults
+    p->imax = 521;
+    p->jmax = 481;
+    p->nk = 5;
+    p->D = 90;
+    p->dm = (19.5 - (-6.5)) / (double)(float)(521 - 1);
+    p->dn = (2.4 - (-2.4)) / (double)(float)(481 - 1);
+    p->dt = 0.01;
+    p->sigmaf = 0.1;
+    p->gamma = 1.4;  // init.f90:361
+    p->Re = 10000.0;
+    p->Pr = 0.71;
+    p->Pe = 1000.0;        // global.f90:38
+    p->alpha_cond = 0.5;   // global.f90:39
+    p->Tref = 273.15;
+    // coefficients.f90:113-123 (N2, Svehla 1995) and :78-79
+    p->A_mi = 0.62526577;
+    p->B_mi = -31.779652;
+    p->C_mi = -1640.7913;
+    p->D_mi = 1.7454992;
+    p->A_k = 0.85372829;
+    p->B_k = 105.18665;
+    p->C_k = -12229.753;
+    p->D_k = 0.48299104;
+    const double T = p->Tref;
+    p->K0 = std::exp(p->A_k * std::log(1.0 * T) + p->B_k / T + p->C_k / T / T + p->D_k);
+    p->MI0 = std::exp(p->A_mi * std::log(1.0 * T) + p->B_mi / T + p->C_mi / T / T + p->D_mi);
+    p->amplitude = 0.8;
+    p->wf = 1.0;
+    p->r0_acous = 0.03;
+    p->m_pt_acous = 0.0;
+    p->n_pt_acous = 0.0;
+    p->precision = NS_PREC_FP64;
+    p->flags = 0;
+    p->device = 0;
+    return NS_OK;
+}
+
+int32_t ns_create(const ns_params *p, ns_ctx **out)
+{
+    if (!p || !out) return fail(NS_ERR_PARAM, "null argument");
+    *out = nullptr;
+    const ns_params &P = *p;
+    if (P.imax < 16 || P.jmax < 16) return fail(NS_ERR_PARAM, "imax and jmax must be >= 16 (got %d x %d)", P.imax, P.jmax);
+    if (P.nk != 4 && P.nk != 5) return fail(NS_ERR_PARAM, "nk must be 4 or 5 (got %d)", P.nk);
+    if (P.D < 0 || 2 * P.D > P.imax || 2 * P.D > P.jmax)
+        return fail(NS_ERR_PARAM, "buffer width D=%d must satisfy 0 <= 2D <= min(imax, jmax)", P.D);
+    if (!(P.dm > 0) || !(P.dn > 0) || !(P.dt > 0)) return fail(NS_ERR_PARAM, "dm, dn, dt must be positive");
+    if (!(P.sigmaf >= 0.0 && P.sigmaf <= 1.0)) return fail(NS_ERR_PARAM, "sigmaf must lie in [0,1]");
+    if (P.precision != NS_PREC_FP64)
+        return fail(NS_ERR_PARAM, "precision mode %d is not available in this build", P.precision);
+    if (!(P.gamma > 1.0) || !(P.Re > 0) || !(P.Pr > 0) || !(P.Pe > 0) || !(P.Tref > 0) || !(P.MI0 > 0) || !(P.K0 > 0))
+        return fail(NS_ERR_PARAM, "gamma, Re, Pr, Pe, Tref, MI0, K0 must be positive (gamma > 1)");
+    const int W = filter_warmup(P.sigmaf);
+    if (W < 0) return fail(NS_ERR_PARAM, "filter recurrence does not decay for sigmaf=%g", P.sigmaf);
+
+    ns_ctx *c = new ns_ctx();
+    c->P = P;
+    c->filt_W = W;
+    c->filt_L = 256;
+    c->pitch = (P.imax + 63) / 64 * 64;
+    c->plane = (size_t)c->pitch * P.jmax;
+    auto bail = [&](int code) {
+        release(c);
+        delete c;
+        return code;
+    };
+    hipError_t e = hipSetDevice(P.device);
+    if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipSetDevice(%d): %s", P.device, hipGetErrorString(e)));
+    int lds = 0;
+    hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, P.device);
+    if (lds < 150 * 1024)
+        return bail(fail(NS_ERR_HIP, "device %d offers %d bytes of LDS per workgroup; the stage kernel needs 150 KiB (gfx950)",
+                         P.device, lds));
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
+    c->own_stream = true;
+    const size_t bytes = (size_t)P.nk * c->plane * sizeof(double);
+    for (double **pp : {&c->S[0], &c->S[1], &c->S[2], &c->Wr, &c->base}) {
+        e = hipMalloc(pp, bytes);
+        if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e)));
+        e = hipMemset(*pp, 0, bytes);
+        if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)));
+    }
+    e = hipMalloc(&c->work, 16 * c->plane * sizeof(double));
+    if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(work): %s", hipGetErrorString(e)));
+    e = hipMalloc(&c->prof, profile_count(P) * sizeof(double));
+    if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(profiles): %s", hipGetErrorString(e)));
+    ProfileLayout L = profile_layout(P);
+    c->metx = c->prof + L.metx;
+    c->sigx = c->prof + L.sigx;
+    c->gx = c->prof + L.gx;
+    c->mety = c->prof + L.mety;
+    c->sigy = c->prof + L.sigy;
+    c->rhob = c->prof + L.rhob;
+    c->psib = c->prof + L.psib;
+    c->gy = c->prof + L.gy;
+    e = hipMalloc(&c->dptrs, 4 * 8 * sizeof(double *));
+    if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(ptrs): %s", hipGetErrorString(e)));
+    e = hipMalloc(&c->dflag, sizeof(int) * 64);
+    if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(flag): %s", hipGetErrorString(e)));
+    for (auto &ev : c->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e)));
+    }
+    *out = c;
+    return NS_OK;
+}
+
+int32_t ns_destroy(ns_ctx *ctx)
+{
+    if (!ctx) return NS_OK;
+    hipSetDevice(ctx->P.device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    release(ctx);
+    delete ctx;
+    return NS_OK;
+}
+
+int32_t ns_set_stream(ns_ctx *c, void *s)
+{
+    if (!c) return fail(NS_ERR_PARAM, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->graph) {
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    if (c->own_stream) hipStreamDestroy(c->stream);
+    if (s) {
+        c->stream = (hipStream_t)s;
+        c->own_stream = false;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    return NS_OK;
+}
+
+int32_t ns_synchronize(ns_ctx *c)
+{
+    if (!c) return fail(NS_ERR_PARAM, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int32_t ns_set_profiles(ns_ctx *c, const double *meshx, const double *meshy, const double *sigmax,
+                        const double *sigmay, const double *m, const double *n, const double *rhob,
+                        const double *psib)
+{
+    if (!c) return fail(NS_ERR_PARAM, "null context");
+    const ns_params &P = c->P;
+    if (!meshx || !meshy || !sigmax || !sigmay || !m || !n || !rhob)
+        return fail(NS_ERR_PARAM, "profile pointer is NULL");
+    if (P.nk >= 5 && !psib) return fail(NS_ERR_PARAM, "psib is required when nk == 5");
+    const int imax = P.imax, jmax = P.jmax, D = P.D;
+    const bool everywhere = P.flags & NS_FLAG_METRIC_EVERYWHERE;
+    std::vector<double> h(profile_count(P), 0.0);
+    const ProfileLayout L = profile_layout(P);
+    double *metx = h.data() + L.metx, *sigx = h.data() + L.sigx, *gx = h.data() + L.gx;
+    double *mety = h.data() + L.mety, *sigy = h.data() + L.sigy, *rb = h.data() + L.rhob;
+    double *pb = h.data() + L.psib, *gy = h.data() + L.gy;
+    const double r2 = P.r0_acous * P.r0_acous;
+    for (int i = 0; i < imax; ++i) {
+        // metricsX (diff.f90:315-350): meshx applies in the x layers only
+        const bool strip = (i + 1 <= D) || (i + 1 >= imax - D + 1);
+        metx[i] = (everywhere || strip) ? meshx[i] : 1.0;
+        sigx[i] = sigmax[i];
+        const double dx = m[i] - P.m_pt_acous;
+        gx[i] = r2 > 0 ? std::exp(-(std::log(2.0) * (dx * dx) / r2)) : 0.0;
+    }
+    for (int j = 0; j < jmax; ++j) {
+        const bool strip = (j + 1 <= D) || (j + 1 >= jmax - D + 1);
+        mety[j] = (everywhere || strip) ? meshy[j] : 1.0;
+        sigy[j] = sigmay[j];
+        rb[j] = rhob[j];
+        pb[j] = psib ? psib[j] : 0.0;
+        const double dy = n[j] - P.n_pt_acous;
+        gy[j] = r2 > 0 ? std::exp(-(std::log(2.0) * (dy * dy) / r2)) : 0.0;
+    }
+    HIPCHK(hipMemcpyAsync(c->prof, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_profiles = true;
+    return NS_OK;
+}
+
+int32_t ns_set_base(ns_ctx *c, const double *Ub)
+{
+    if (!c || !Ub) return fail(NS_ERR_PARAM, "null argument");
+    int rc = upload_aos(c, c->base, Ub, c->P.nk);
+    if (rc == NS_OK) c->have_base = true;
+    return rc;
+}
+
+int32_t ns_put_state(ns_ctx *c, const double *U)
+{
+    if (!c || !U) return fail(NS_ERR_PARAM, "null argument");
+    int rc = upload_aos(c, c->S[0], U, c->P.nk);
+    if (rc == NS_OK) c->have_state = true;
+    return rc;
+}
+
+int32_t ns_get_state(ns_ctx *c, double *U)
+{
+    if (!c || !U) return fail(NS_ERR_PARAM, "null argument");
+    return download_aos(c, c->S[0], U, c->P.nk);
+}
+
+int32_t ns_probe(ns_ctx *c, int32_t npts, const int32_t *ij, double *out)
+{
+    if (!c || !ij || !out || npts < 0) return fail(NS_ERR_PARAM, "bad argument");
+    if (npts == 0) return NS_OK;
+    for (int t = 0; t < npts; ++t)
+        if (ij[2 * t] < 1 || ij[2 * t] > c->P.imax || ij[2 * t + 1] < 1 || ij[2 * t + 1] > c->P.jmax)
+            return fail(NS_ERR_PARAM, "probe %d at (%d,%d) lies outside the grid", t, ij[2 * t], ij[2 * t + 1]);
+    const int nk = c->P.nk;
+    int *dij = nullptr;
+    double *dout = nullptr;
+    HIPCHK(hipMalloc(&dij, sizeof(int) * 2 * npts));
+    HIPCHK(hipMalloc(&dout, sizeof(double) * npts * nk));
+    int rc = set_ptr_table(c, 0, c->S[0], nk);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(dij, ij, sizeof(int) * 2 * npts, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_probe((const double *const *)c->dptrs, nk, c->pitch, npts, dij, dout, c->stream));
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * npts * nk, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipFree(dij);
+    hipFree(dout);
+    return NS_OK;
+}
+
+int32_t ns_check_finite(ns_ctx *c)
+{
+    if (!c) return fail(NS_ERR_PARAM, "null context");
+    int rc = set_ptr_table(c, 0, c->S[0], c->P.nk);
+    if (rc) return rc;
+    int zero = 0, flag = 0;
+    HIPCHK(hipMemcpyAsync(c->dflag, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_finite((const double *const *)c->dptrs, c->P.nk, c->P.imax, c->P.jmax, c->pitch, c->dflag, c->stream));
+    HIPCHK(hipMemcpyAsync(&flag, c->dflag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (flag) return fail(NS_ERR_NONFINITE, "state holds NaN or Inf");
+    return NS_OK;
+}
+
+int32_t ns_rk_euler(ns_ctx *c, int32_t iter)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    rc = enqueue_step(c, iter, false);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int32_t ns_rk_steps(ns_ctx *c, int32_t iter0, int32_t nsteps)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(NS_ERR_PARAM, "nsteps < 0");
+    if (c->timing || c->P.amplitude != 0.0) {
+        // per-step launches: the source amplitude changes with iter
+        for (int s = 0; s < nsteps; ++s) {
+            rc = enqueue_step(c, iter0 + s, c->timing);
+            if (rc) return rc;
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return NS_OK;
+    }
+    if (!c->graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_step(c, iter0, false);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(NS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(NS_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// operator-level mirrors
+// ---------------------------------------------------------------------------
+static int deriv_common(ns_ctx *c, int nk, double *dU, const double *U, bool ydir)
+{
+    if (!c || !dU || !U) return fail(NS_ERR_PARAM, "null argument");
+    if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
+    if (nk < 1 || nk > 8) return fail(NS_ERR_PARAM, "nk must lie in 1..8");
+    double *in = c->work, *outb = c->work + 8 * c->plane;
+    int rc = upload_aos(c, in, U, nk);
+    if (rc) return rc;
+    if ((rc = set_ptr_table(c, 0, in, nk)) || (rc = set_ptr_table(c, 1, outb, nk))) return rc;
+    HIPCHK(launch_deriv((const double *const *)c->dptrs, (double *const *)(c->dptrs + 8), nk, c->P.imax, c->P.jmax,
+                        c->pitch, ydir, ydir ? c->mety : c->metx, ydir ? 1.0 / c->P.dn : 1.0 / c->P.dm, c->stream));
+    return download_aos(c, outb, dU, nk);
+}
+
+int32_t ns_deropm(ns_ctx *c, int32_t nk, double *dU, const double *U) { return deriv_common(c, nk, dU, U, false); }
+int32_t ns_deropn(ns_ctx *c, int32_t nk, double *dU, const double *U) { return deriv_common(c, nk, dU, U, true); }
+
+int32_t ns_transport(ns_ctx *c, const double *T, double *MI, double *K1)
+{
+    if (!c || !T || !MI || !K1) return fail(NS_ERR_PARAM, "null argument");
+    const size_t n = (size_t)c->P.imax * c->P.jmax;
+    double *dT = c->work, *dM = c->work + n, *dK = c->work + 2 * n;
+    HIPCHK(hipMemcpyAsync(dT, T, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_transport(dT, dM, dK, n, make_transport(c->P), c->stream));
+    HIPCHK(hipMemcpyAsync(MI, dM, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(K1, dK, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int32_t ns_rhs(ns_ctx *c, double *dUdt, const double *U, double ct, int32_t iter)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!dUdt || !U) return fail(NS_ERR_PARAM, "null argument");
+    const ns_params &P = c->P;
+    double *in = c->work, *outb = c->work + 8 * c->plane;
+    if ((rc = upload_aos(c, in, U, P.nk))) return rc;
+    double amp = P.amplitude == 0.0 ? 0.0 : P.amplitude * std::sin(P.wf * (((double)iter + ct) * P.dt));
+    StageArgs<double> A = stage_args(c, 0, in, outb, amp);
+    for (int k = 0; k < P.nk; ++k) A.dudt[k] = plane_ptr(outb, c, k);
+    HIPCHK(launch_stage(A, P.nk, !(P.flags & NS_FLAG_INVISCID), true, c->stream));
+    return download_aos(c, outb, dUdt, P.nk);
+}
+
+int32_t ns_rk_steep(ns_ctx *c, double alphat, double betat, double *U, const double *dudt, double *w)
+{
+    if (!c || !U || !dudt || !w) return fail(NS_ERR_PARAM, "null argument");
+    const int nk = c->P.nk;
+    double *dU = c->work, *dD = c->work + 5 * c->plane, *dW = c->work + 10 * c->plane;
+    int rc;
+    if ((rc = upload_aos(c, dU, U, nk)) || (rc = upload_aos(c, dD, dudt, nk)) || (rc = upload_aos(c, dW, w, nk)))
+        return rc;
+    if ((rc = set_ptr_table(c, 0, dU, nk)) || (rc = set_ptr_table(c, 1, dD, nk)) || (rc = set_ptr_table(c, 2, dW, nk)))
+        return rc;
+    HIPCHK(launch_rk_steep((double *const *)c->dptrs, (const double *const *)(c->dptrs + 8),
+                           (double *const *)(c->dptrs + 16), nk, c->P.imax, c->P.jmax, c->pitch, alphat, betat,
+                           c->P.dt, c->stream));
+    if ((rc = download_aos(c, dU, U, nk))) return rc;
+    return download_aos(c, dW, w, nk);
+}
+
+int32_t ns_bufferxy(ns_ctx *c, double *U)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!U) return fail(NS_ERR_PARAM, "null argument");
+    const int nk = c->P.nk;
+    double *dU = c->work;
+    if ((rc = upload_aos(c, dU, U, nk))) return rc;
+    if ((rc = set_ptr_table(c, 0, dU, nk)) || (rc = set_ptr_table(c, 1, c->base, nk))) return rc;
+    HIPCHK(launch_buffer((double *const *)c->dptrs, (const double *const *)(c->dptrs + 8), nk, c->P.imax, c->P.jmax,
+                         c->pitch, c->P.D, c->sigx, c->sigy, c->stream));
+    return download_aos(c, dU, U, nk);
+}
+
+static int filter_common(ns_ctx *c, double *U, bool xdir)
+{
+    if (!c || !U) return fail(NS_ERR_PARAM, "null argument");
+    const int nk = c->P.nk;
+    double *in = c->work, *outb = c->work + 8 * c->plane;
+    int rc;
+    if ((rc = upload_aos(c, in, U, nk))) return rc;
+    HIPCHK(launch_filter(filter_args(c, xdir, in, outb, false), xdir, c->stream));
+    return download_aos(c, outb, U, nk);
+}
+
+int32_t ns_filter_x(ns_ctx *c, double *U) { return filter_common(c, U, true); }
+int32_t ns_filter_y(ns_ctx *c, double *U) { return filter_common(c, U, false); }
+
+int32_t ns_ccpml(ns_ctx *c, double *U)
+{
+    if (!c || !U) return fail(NS_ERR_PARAM, "null argument");
+    if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
+    const ns_params &P = c->P;
+    const int nk = P.nk;
+    double *dU = c->work;
+    int rc;
+    if ((rc = upload_aos(c, dU, U, nk))) return rc;
+    // rows j=1, jmax of interior columns come from the y-filter kernel's edge
+    // epilogue (run here at sigma = 0, which leaves every line unchanged),
+    // then columns i=1, imax with the corners from edge_i_kernel.
+    FilterArgs<double> F = filter_args(c, false, dU, c->work + 8 * c->plane, true);
+    F.sigma = 0.0;
+    HIPCHK(launch_filter(F, false, c->stream));
+    HIPCHK(launch_edge_i(edge_args(c, c->work + 8 * c->plane), c->stream));
+    return download_aos(c, c->work + 8 * c->plane, U, nk);
+}
+
+int32_t ns_acousticfont(ns_ctx *c, double ct, int32_t iter, double *s_acou)
+{
+    if (!c || !s_acou) return fail(NS_ERR_PARAM, "null argument");
+    if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
+    const ns_params &P = c->P;
+    const double amp = P.amplitude * std::sin(P.wf * (((double)iter + ct) * P.dt));
+    const size_t n = (size_t)P.imax * P.jmax;
+    HIPCHK(launch_source(c->work, P.imax, P.jmax, amp, c->gx, c->gy, c->stream));
+    HIPCHK(hipMemcpyAsync(s_acou, c->work, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int32_t ns_enable_timing(ns_ctx *c, int32_t on)
+{
+    if (!c) return fail(NS_ERR_PARAM, "null context");
+    c->timing = on != 0;
+    for (int t = 0; t < TC_N; ++t) {
+        c->tsum[t] = 0.0;
+        c->tcount[t] = 0;
+    }
+    return NS_OK;
+}
+
+int32_t ns_kernel_time(ns_ctx *c, const char *name, double *avg_ms, int32_t *count)
+{
+    if (!c || !name || !avg_ms || !count) return fail(NS_ERR_PARAM, "null argument");
+    for (int t = 0; t < TC_N; ++t)
+        if (std::strcmp(name, TC_NAMES[t]) == 0) {
+            *count = (int32_t)c->tcount[t];
+            *avg_ms = c->tcount[t] ? c->tsum[t] / c->tcount[t] : 0.0;
+            return NS_OK;
+        }
+    return fail(NS_ERR_PARAM, "unknown kernel class '%s'", name);
+}
+
+}  // extern "C"

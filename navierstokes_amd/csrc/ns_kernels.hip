@@ -1,0 +1,753 @@
+// ns_kernels.hip -- gfx950 kernels of the mathe1s/NavierStokes time-advance
+// hot path, FP64 (reference semantics).
+//
+// Per LDDRK stage (rk.f90:52-73) the device runs
+//   stage_kernel   : RHS (rhs.f90:10-46) + rk_steep (rk.f90:153-169)
+//                    + bufferxy (rk.f90:77-151), one LDS-tiled sweep
+//   filter_x_kernel: lddfilterx (filtering.f90:9-96)
+//   filter_y_kernel: lddfiltery (filtering.f90:98-190) + ccpml rows j=1,jmax
+//   edge_i_kernel  : ccpml columns i=1,imax incl. corners (cc.f90:6-68)
+// State planes are structure-of-arrays, row-major with i fastest (pitch).
+#include <hip/hip_runtime.h>
+
+#include "ns_kernels.h"
+
+namespace nsamd {
+
+// ---------------------------------------------------------------------------
+// Stage kernel: LDS-tiled RHS with nested stencils, RK update and buffer.
+//
+// Tile TX x TY outputs.  The U window holds the tile plus a 6-point halo
+// (3 for the first derivatives, 3 more for the derivatives of tau and k dT);
+// at the domain edges the one-sided stencils reach 6 points inward, handled by
+// shifting the window/level-1 boxes (see tile_ranges).
+// ---------------------------------------------------------------------------
+constexpr int STX = 64;
+
+struct Range {
+    int l0, l1, w0, w1;  // level-1 box [l0,l1), U window [w0,w1) (global, 0-based)
+};
+
+__device__ __forceinline__ Range tile_ranges(int t0, int t1, int n)
+{
+    Range r;
+    r.l0 = max(0, t0 - 3);
+    if (t1 - 1 >= n - 3) r.l0 = min(r.l0, n - 7);
+    r.l1 = min(n, t1 + 3);
+    if (t0 <= 2) r.l1 = max(r.l1, 7);
+    r.w0 = max(0, r.l0 - 3);
+    if (r.l1 - 1 >= n - 3) r.w0 = min(r.w0, n - 7);
+    r.w1 = min(n, r.l1 + 3);
+    if (r.l0 <= 2) r.w1 = max(r.w1, 7);
+    return r;
+}
+
+template <typename R, int NK, bool VISC, int TY, bool RHS_ONLY>
+__global__ __launch_bounds__(256) void stage_kernel(StageArgs<R> A)
+{
+    constexpr int TX = STX;
+    constexpr int WX = TX + 12, WY = TY + 12;
+    constexpr int LX = TX + 6, LY = TY + 6;
+    extern __shared__ char smem_raw[];
+    R *sU = reinterpret_cast<R *>(smem_raw);  // NK x WY x WX
+    R *sT = sU + NK * WY * WX;                // WY x WX (VISC)
+    R *sL = sT + (VISC ? WY * WX : 0);        // NL x LY x LX
+
+    const int imax = A.imax, jmax = A.jmax, pitch = A.pitch;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i1 = min(i0 + TX, imax), j1 = min(j0 + TY, jmax);
+    const Range rx = tile_ranges(i0, i1, imax);
+    const Range ry = tile_ranges(j0, j1, jmax);
+    const int tid = threadIdx.x;
+
+    // phase 1: U window (and T = gamma p / rho, rhs.f90:20)
+    const int wxn = rx.w1 - rx.w0, wyn = ry.w1 - ry.w0;
+    for (int idx = tid; idx < wyn * WX; idx += 256) {
+        const int wy = idx / WX, wx = idx - wy * WX;
+        if (wx < wxn) {
+            const size_t g = (size_t)(ry.w0 + wy) * pitch + (rx.w0 + wx);
+#pragma unroll
+            for (int k = 0; k < NK; ++k) sU[(k * WY + wy) * WX + wx] = A.U[k][g];
+            if (VISC) sT[wy * WX + wx] = A.gamma * sU[(3 * WY + wy) * WX + wx] / sU[(0 * WY + wy) * WX + wx];
+        }
+    }
+    __syncthreads();
+
+    const int ox = rx.w0, oy = ry.w0;
+    auto UX = [&](int k, int gy) {  // accessor along x at global row gy
+        const R *row = sU + (k * WY + (gy - oy)) * WX;
+        return [row, ox](int q) { return row[q - ox]; };
+    };
+    auto UYc = [&](int k, int gx) {  // accessor along y at global column gx
+        const R *col = sU + k * WY * WX + (gx - ox);
+        return [col, oy](int q) { return col[(q - oy) * WX]; };
+    };
+
+    // phase 2: level-1 quantities (viscous stresses, k grad T, species flux)
+    if constexpr (VISC) {
+        const int lxn = rx.l1 - rx.l0, lyn = ry.l1 - ry.l0;
+        for (int idx = tid; idx < lyn * LX; idx += 256) {
+            const int ly = idx / LX, lx = idx - ly * LX;
+            if (lx >= lxn) continue;
+            const int gx = rx.l0 + lx, gy = ry.l0 + ly;
+            const R mx = A.metx[gx], my = A.mety[gy];
+            const R ux = drp_deriv<R>(UX(1, gy), gx, imax, A.inv_dm) * mx;
+            const R vx = drp_deriv<R>(UX(2, gy), gx, imax, A.inv_dm) * mx;
+            const R uy = drp_deriv<R>(UYc(1, gx), gy, jmax, A.inv_dn) * my;
+            const R vy = drp_deriv<R>(UYc(2, gx), gy, jmax, A.inv_dn) * my;
+            const R *trow = sT + (gy - oy) * WX;
+            const R *tcol = sT + (gx - ox);
+            const R Tx = drp_deriv<R>([trow, ox](int q) { return trow[q - ox]; }, gx, imax, A.inv_dm) * mx;
+            const R Ty = drp_deriv<R>([tcol, oy](int q) { return tcol[(q - oy) * WX]; }, gy, jmax, A.inv_dn) * my;
+            R mu, kk;
+            transport_eval<R>(A.tc, trow[gx - ox], mu, kk);
+            const R c23mu = R(2.0 / 3.0) * mu;
+            const R div = ux + vy;
+            const int o = ly * LX + lx;
+            sL[0 * LY * LX + o] = -(c23mu * div) + R(2) * mu * ux;  // eqeuler.f90:37
+            sL[1 * LY * LX + o] = mu * (uy + vx);                    // :38
+            sL[2 * LY * LX + o] = -(c23mu * div) + R(2) * mu * vy;  // :39
+            sL[3 * LY * LX + o] = Tx * kk;                           // derivs.f90:55
+            sL[4 * LY * LX + o] = Ty * kk;                           // :54
+            if constexpr (NK >= 5) {
+                const R psx = drp_deriv<R>(UX(4, gy), gx, imax, A.inv_dm) * mx;
+                const R psy = drp_deriv<R>(UYc(4, gx), gy, jmax, A.inv_dn) * my;
+                const R rho = UX(0, gy)(gx);
+                sL[5 * LY * LX + o] = rho * A.alpha_cond * (psx + psy);  // derivs.f90:40
+            }
+        }
+        __syncthreads();
+    }
+
+    // phase 3: RHS at the tile's points, then the stage update
+    const int tx = tid & 63;
+    const int gx = i0 + tx;
+    if (gx >= i1) return;
+    const R mx = A.metx[gx];
+    for (int gy = j0 + (tid >> 6); gy < j1; gy += 4) {
+        const bool interior = gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2;
+        const R my = A.mety[gy];
+        R uc[NK], fx[NK], fy[NK];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            uc[k] = UX(k, gy)(gx);
+            fx[k] = drp_deriv<R>(UX(k, gy), gx, imax, A.inv_dm) * mx;
+            fy[k] = drp_deriv<R>(UYc(k, gx), gy, jmax, A.inv_dn) * my;
+        }
+        const R rho = uc[0], u = uc[1], v = uc[2], p = uc[3];
+        const R irho = R(1) / rho;
+        const R src = (A.amp_src != R(0)) ? A.amp_src * (A.gx[gx] * A.gy[gy]) : R(0);
+        R f[NK];
+        // eqeuler.f90:63-66
+        f[0] = -(u * fx[0] + rho * fx[1] + v * fy[0] + rho * fy[2]);
+        if constexpr (VISC) {
+            const int lx0 = rx.l0, ly0 = ry.l0;
+            auto LXa = [&](int l) {
+                const R *row = sL + (l * LY + (gy - ly0)) * LX;
+                return [row, lx0](int q) { return row[q - lx0]; };
+            };
+            auto LYa = [&](int l) {
+                const R *col = sL + l * LY * LX + (gx - lx0);
+                return [col, ly0](int q) { return col[(q - ly0) * LX]; };
+            };
+            const R txx_x = drp_deriv<R>(LXa(0), gx, imax, A.inv_dm) * mx;
+            const R txy_x = drp_deriv<R>(LXa(1), gx, imax, A.inv_dm) * mx;
+            const R ktx_x = drp_deriv<R>(LXa(3), gx, imax, A.inv_dm) * mx;
+            const R txy_y = drp_deriv<R>(LYa(1), gy, jmax, A.inv_dn) * my;
+            const R tyy_y = drp_deriv<R>(LYa(2), gy, jmax, A.inv_dn) * my;
+            const R kty_y = drp_deriv<R>(LYa(4), gy, jmax, A.inv_dn) * my;
+            const int o = (gy - ry.l0) * LX + (gx - rx.l0);
+            const R t0 = sL[0 * LY * LX + o], t1 = sL[1 * LY * LX + o], t2 = sL[2 * LY * LX + o];
+            const R vre = A.inv_Re * irho;
+            // eqeuler.f90:68-92
+            f[1] = -(u * fx[1] + irho * fx[3] + v * fy[1] - vre * (txx_x + txy_y));
+            f[2] = -(u * fx[2] + v * fy[2] + irho * fy[3] - vre * (txy_x + tyy_y));
+            f[3] = -(A.gamma * p * fx[1] + u * fx[3] + A.gamma * p * fy[2] + v * fy[3] - src -
+                     A.gm1_Re * (fx[1] * t0 + fx[2] * t1 + fy[1] * t1 + fy[2] * t2) -
+                     A.inv_RePr * (ktx_x + kty_y));
+            if constexpr (NK >= 5) {
+                const R psd_x = drp_deriv<R>(LXa(5), gx, imax, A.inv_dm) * mx;
+                const R psd_y = drp_deriv<R>(LYa(5), gy, jmax, A.inv_dn) * my;
+                f[4] = -(u * fx[4] + v * fy[4] - A.inv_Pe * irho * (psd_x + psd_y));  // :94-95
+            }
+        } else {
+            f[1] = -(u * fx[1] + irho * fx[3] + v * fy[1]);
+            f[2] = -(u * fx[2] + v * fy[2] + irho * fy[3]);
+            f[3] = -(A.gamma * p * fx[1] + u * fx[3] + A.gamma * p * fy[2] + v * fy[3] - src);
+            if constexpr (NK >= 5) f[4] = -(u * fx[4] + v * fy[4]);
+        }
+        const size_t g = (size_t)gy * pitch + gx;
+        if constexpr (RHS_ONLY) {
+#pragma unroll
+            for (int k = 0; k < NK; ++k) A.dudt[k][g] = f[k];
+        } else {
+            const bool ybuf = gy < A.D || gy >= jmax - A.D;
+            const bool xbuf = gx < A.D || gx >= imax - A.D;
+            const R sy = A.sigy[gy], sx = A.sigx[gx];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                R un = uc[k];
+                if (interior) {  // rk.f90:160-167
+                    const R wn = A.first ? A.dt * f[k] : A.alpha * A.W[k][g] + A.dt * f[k];
+                    if (!A.last) A.W[k][g] = wn;
+                    un = uc[k] + A.beta * wn;
+                }
+                if (ybuf || xbuf) {  // rk.f90:81-108 (y layer first, then x)
+                    const R ub = A.base[k][g];
+                    if (ybuf) un = ub + sy * (un - ub);
+                    if (xbuf) un = ub + sx * (un - ub);
+                }
+                A.Uout[k][g] = un;
+            }
+        }
+    }
+}
+
+template <typename R>
+static size_t stage_smem(int NK, bool visc, int TY)
+{
+    const int WX = STX + 12, WY = TY + 12, LX = STX + 6, LY = TY + 6;
+    const int NL = visc ? (NK >= 5 ? 6 : 5) : 0;
+    return sizeof(R) * ((size_t)NK * WY * WX + (visc ? (size_t)WY * WX : 0) + (size_t)NL * LY * LX);
+}
+
+template <typename R, int NK, bool VISC, bool RHS_ONLY>
+static hipError_t launch_stage_t(const StageArgs<R> &A, hipStream_t s)
+{
+    constexpr int TY = (NK >= 5 && VISC) ? 8 : 16;
+    dim3 grid((A.imax + STX - 1) / STX, (A.jmax + TY - 1) / TY);
+    size_t smem = stage_smem<R>(NK, VISC, TY);
+    auto kern = stage_kernel<R, NK, VISC, TY, RHS_ONLY>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, grid, dim3(256), smem, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, hipStream_t s)
+{
+    if (nk == 4) {
+        if (visc) return rhs_only ? launch_stage_t<double, 4, true, true>(A, s) : launch_stage_t<double, 4, true, false>(A, s);
+        return rhs_only ? launch_stage_t<double, 4, false, true>(A, s) : launch_stage_t<double, 4, false, false>(A, s);
+    }
+    if (visc) return rhs_only ? launch_stage_t<double, 5, true, true>(A, s) : launch_stage_t<double, 5, true, false>(A, s);
+    return rhs_only ? launch_stage_t<double, 5, false, true>(A, s) : launch_stage_t<double, 5, false, false>(A, s);
+}
+
+// ---------------------------------------------------------------------------
+// In-place selective filter as a chunked sweep with look-back.
+//
+// The reference sweeps each line in place (filtering.f90): a point's update
+// reads the already-updated points behind it, so the interior is a linear
+// recurrence with 6 feedback taps sigma*d(-1..-6).  Its homogeneous response
+// decays like 0.33^n at sigma=0.2, so a chunk starting at position a can start
+// the recurrence W points earlier from unfiltered values: the error left at a
+// is below 1e-16 of the filter correction for the W chosen by the host.
+// Chunks that start within W+12 of the left end run the exact sweep from the
+// start (left block + interior); the chunk that holds the right end also runs
+// the descending right block (filtering.f90:81-94).
+//
+// One lane filters one line.  Window w[13] holds positions q-6..q+6 with
+// slot (u+s)%13 <-> position q-6+s at unroll step u, so no register moves.
+// ---------------------------------------------------------------------------
+
+template <int ROT, typename R>
+__device__ __forceinline__ void unrot13(const R (&w)[13], R (&v)[13])
+{
+#pragma unroll
+    for (int s = 0; s < 13; ++s) v[s] = w[(ROT + s) % 13];
+}
+
+template <typename R>
+__device__ __forceinline__ void unrotate(int rot, const R (&w)[13], R (&v)[13])
+{
+    switch (rot) {
+    case 0: unrot13<0>(w, v); break;
+    case 1: unrot13<1>(w, v); break;
+    case 2: unrot13<2>(w, v); break;
+    case 3: unrot13<3>(w, v); break;
+    case 4: unrot13<4>(w, v); break;
+    case 5: unrot13<5>(w, v); break;
+    case 6: unrot13<6>(w, v); break;
+    case 7: unrot13<7>(w, v); break;
+    case 8: unrot13<8>(w, v); break;
+    case 9: unrot13<9>(w, v); break;
+    case 10: unrot13<10>(w, v); break;
+    case 11: unrot13<11>(w, v); break;
+    default: unrot13<12>(w, v); break;
+    }
+}
+
+struct LineEnds {
+    double v1, v2;      // final values at positions 1, 2 (exact-start chunk)
+    double vn2, vn3;    // final values at positions n-2, n-3 (last chunk)
+};
+
+// IO policy contract: load(p) for strictly increasing p (p < n),
+// emit(e, v) for strictly increasing e, finish() at the end.
+template <typename R, bool YDIR, typename IO>
+__device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b, int W, LineEnds &ends)
+{
+    const bool last = (b == n);
+    const bool exact = (a <= W + 12);
+    R w[13];
+    int q;
+    if (exact) {
+#pragma unroll
+        for (int s = 0; s < 13; ++s) w[s] = io.load(s);
+        // left block, ascending (filtering.f90:36-49)
+#pragma unroll
+        for (int qq = 1; qq <= 6; ++qq) {
+            R Du = R(filt_dfb<YDIR>(0)) * w[qq - 1];
+#pragma unroll
+            for (int s = 1; s < 7; ++s) Du = Du + R(filt_dfb<YDIR>(s)) * w[qq - 1 + s];
+            w[qq] = w[qq] - sigma * Du;
+        }
+        ends.v1 = (double)w[1];
+        ends.v2 = (double)w[2];
+        q = 6;
+    } else {
+        const int q0 = a - W;
+#pragma unroll
+        for (int s = 0; s < 13; ++s) w[s] = io.load(q0 - 6 + s);
+        q = q0;
+    }
+    const int qe = last ? n - 7 : b - 1;
+    // loads for the next 13 steps are issued one group ahead
+    R nb[13];
+#pragma unroll
+    for (int u = 0; u < 13; ++u) nb[u] = (q + 7 + u < n) ? io.load(q + 7 + u) : R(0);
+    int rot = 0;
+    for (;;) {
+        R cur[13];
+#pragma unroll
+        for (int u = 0; u < 13; ++u) cur[u] = nb[u];
+        const int qn = q + 13;
+#pragma unroll
+        for (int u = 0; u < 13; ++u) nb[u] = (qn + 7 + u < n && qn + u <= qe) ? io.load(qn + 7 + u) : R(0);
+        bool done = false;
+#pragma unroll
+        for (int u = 0; u < 13; ++u) {
+            if (!done) {
+                // interior 13-point update (filtering.f90:69-79)
+                R Du = R(filt_d(0)) * w[u % 13];
+#pragma unroll
+                for (int s = 1; s < 13; ++s) Du = Du + R(filt_d(s)) * w[(u + s) % 13];
+                w[(u + 6) % 13] = w[(u + 6) % 13] - sigma * Du;
+                if (q == qe) {
+                    rot = u;
+                    done = true;
+                } else {
+                    io.emit(q - 6, w[u % 13]);
+                    w[u % 13] = cur[u];
+                    ++q;
+                }
+            }
+        }
+        if (done) break;
+    }
+    R v[13];  // positions qe-6 .. qe+6
+    unrotate<R>(rot, w, v);
+    if (last) {
+        // right block, descending, mirrored stencil (filtering.f90:81-94)
+#pragma unroll
+        for (int qq = 11; qq >= 6; --qq) {
+            R Du = R(filt_dfb<YDIR>(0)) * v[qq + 1];
+#pragma unroll
+            for (int s = 0; s <= 5; ++s) Du = Du + R(filt_dfb<YDIR>(s + 1)) * v[qq - s];
+            v[qq] = v[qq] - sigma * Du;
+        }
+        ends.vn2 = (double)v[11];
+        ends.vn3 = (double)v[10];
+#pragma unroll
+        for (int s = 0; s < 13; ++s) io.emit(qe - 6 + s, v[s]);
+    } else {
+#pragma unroll
+        for (int s = 0; s < 7; ++s) io.emit(qe - 6 + s, v[s]);
+    }
+    io.finish();
+}
+
+// chunk bounds: chunks of length L; a chunk that would end within 16 points
+// of the line end absorbs the rest; a chunk starting there is empty.
+__device__ __forceinline__ bool chunk_bounds(int c, int L, int n, int &a, int &b)
+{
+    a = c * L;
+    b = min(a + L, n);
+    if (a > 0 && a > n - 16) return false;
+    if (b > n - 16) b = n;
+    return true;
+}
+
+// x direction: lanes are rows; LDS-staged 64x16 tiles keep global access
+// coalesced (lanes read/write 16-column row segments).
+constexpr int FX_T = 16;
+
+template <typename R>
+struct XIO {
+    const R *in;
+    R *out;
+    int pitch, n, m, row0, a, b, lane;
+    R (*tin)[FX_T + 1];
+    R (*tout)[FX_T + 1];
+    int tc;  // start of the tile in LDS
+    int oc;  // start of the output tile being filled (-1 none)
+    R pre[FX_T];
+
+    __device__ __forceinline__ void fetch(int t)  // issue loads of tile t into registers
+    {
+        const int c = lane & 15, rs = lane >> 4;
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+            const int gj = row0 + rr * 4 + rs, gi = t + c;
+            pre[rr] = (gj < m && gi < n && t >= 0) ? in[(size_t)gj * pitch + gi] : R(0);
+        }
+    }
+    __device__ __forceinline__ void commit()
+    {
+        const int c = lane & 15, rs = lane >> 4;
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) tin[rr * 4 + rs][c] = pre[rr];
+        __syncthreads();
+    }
+    __device__ __forceinline__ void start(int p0)
+    {
+        tc = p0 & ~(FX_T - 1);
+        fetch(tc);
+        commit();
+        fetch(tc + FX_T);
+        oc = -1;
+    }
+    __device__ __forceinline__ R load(int p)
+    {
+        if (p >= tc + FX_T) {  // uniform: all lanes stream the same positions
+            commit();
+            tc += FX_T;
+            fetch(tc + FX_T);
+        }
+        return tin[lane][p - tc];
+    }
+    __device__ __forceinline__ void flush()
+    {
+        if (oc < 0) return;
+        const int c = lane & 15, rs = lane >> 4;
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+            const int gj = row0 + rr * 4 + rs, gi = oc + c;
+            if (gj < m && gi >= a && gi < b) out[(size_t)gj * pitch + gi] = tout[rr * 4 + rs][c];
+        }
+        __syncthreads();
+    }
+    __device__ __forceinline__ void emit(int e, R v)
+    {
+        const int t = e & ~(FX_T - 1);
+        if (t != oc) {
+            flush();
+            oc = t;
+        }
+        tout[lane][e - oc] = v;
+    }
+    __device__ __forceinline__ void finish() { flush(); }
+};
+
+template <typename R>
+__global__ __launch_bounds__(64) void filter_x_kernel(FilterArgs<R> F)
+{
+    __shared__ R tin[64][FX_T + 1];
+    __shared__ R tout[64][FX_T + 1];
+    int a, b;
+    if (!chunk_bounds(blockIdx.x, F.L, F.n, a, b)) return;
+    const int k = blockIdx.z;
+    XIO<R> io;
+    io.in = F.in[k];
+    io.out = F.out[k];
+    io.pitch = F.pitch;
+    io.n = F.n;
+    io.m = F.m;
+    io.row0 = blockIdx.y * 64;
+    io.a = a;
+    io.b = b;
+    io.lane = threadIdx.x;
+    io.tin = tin;
+    io.tout = tout;
+    const bool exact = (a <= F.W + 12);
+    io.start(exact ? 0 : a - F.W - 6);
+    LineEnds ends;
+    filter_line<R, false>(io, F.sigma, F.n, a, b, F.W, ends);
+}
+
+// y direction: lanes are columns; rows are contiguous across lanes.
+template <typename R>
+struct YIO {
+    const R *in;
+    R *out;
+    int pitch, col, a, b;
+    bool active;
+    __device__ __forceinline__ R load(int p) { return active ? in[(size_t)p * pitch + col] : R(0); }
+    __device__ __forceinline__ void emit(int e, R v)
+    {
+        if (active && e >= a && e < b) out[(size_t)e * pitch + col] = v;
+    }
+    __device__ __forceinline__ void finish() {}
+};
+
+template <typename R>
+__global__ __launch_bounds__(64) void filter_y_kernel(FilterArgs<R> F)
+{
+    int a, b;
+    if (!chunk_bounds(blockIdx.y, F.L, F.n, a, b)) return;
+    const int k = blockIdx.z;
+    const int col = blockIdx.x * 64 + threadIdx.x;
+    YIO<R> io;
+    io.in = F.in[k];
+    io.out = F.out[k];
+    io.pitch = F.pitch;
+    io.col = col;
+    io.a = a;
+    io.b = b;
+    io.active = col < F.m;
+    LineEnds ends;
+    filter_line<R, true>(io, F.sigma, F.n, a, b, F.W, ends);
+    // ccpml rows j=1 and j=jmax for columns 2..imax-1 (cc.f90:46-59)
+    if (F.edge_bc && io.active && col >= 1 && col <= F.m - 2) {
+        R *o = F.out[k];
+        const int jm = F.n;
+        if (a == 0) {
+            R val = (k < 3) ? R(2.0) * R(ends.v1) - R(ends.v2) : (k == 3 ? R(1.0) / F.gamma : R(0.0));
+            o[col] = val;
+        }
+        if (b == jm) {
+            R val = (k < 3) ? R(2.0) * R(ends.vn2) - R(ends.vn3) : (k == 3 ? R(1.0) / F.gamma : R(1.0));
+            o[(size_t)(jm - 1) * F.pitch + col] = val;
+        }
+    }
+}
+
+hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s)
+{
+    const int nchunks = (F.n + F.L - 1) / F.L;
+    if (xdir) {
+        dim3 grid(nchunks, (F.m + 63) / 64, F.nk);
+        hipLaunchKernelGGL(filter_x_kernel<double>, grid, dim3(64), 0, s, F);
+    } else {
+        dim3 grid((F.m + 63) / 64, nchunks, F.nk);
+        hipLaunchKernelGGL(filter_y_kernel<double>, grid, dim3(64), 0, s, F);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// ccpml columns i=1 (inflow) and i=imax (outflow) for every row, with the
+// corner rows j=1, jmax computed from those columns' values (cc.f90:23-59).
+// ---------------------------------------------------------------------------
+__global__ void edge_i_kernel(EdgeArgs E)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int side = blockIdx.y;  // 0: inflow, 1: outflow
+    if (j >= E.jmax) return;
+    const int imax = E.imax, jmax = E.jmax, pitch = E.pitch;
+    const double ig = 1.0 / E.gamma;
+    auto at = [&](int k, int i, int jj) { return E.U[k][(size_t)jj * pitch + i]; };
+    // value of the column after the inflow/outflow assignment at row jj
+    auto colval = [&](int k, int jj) -> double {
+        if (side == 0) {
+            switch (k) {
+            case 0: return E.rhob[jj];
+            case 1: return 2.0 * at(1, 1, jj) - at(1, 2, jj);
+            case 2: return 0.0;
+            case 3: return ig;
+            default: return E.psib ? E.psib[jj] : 0.0;
+            }
+        }
+        if (k == 3) return ig;
+        return 2.0 * at(k, imax - 2, jj) - at(k, imax - 3, jj);
+    };
+    const int i = side == 0 ? 0 : imax - 1;
+    double vals[NKMAX];
+    for (int k = 0; k < E.nk; ++k) {
+        double v;
+        if (j == 0) {  // lower boundary overwrites the corner (cc.f90:54-59)
+            v = (k < 3) ? 2.0 * colval(k, 1) - colval(k, 2) : (k == 3 ? ig : 0.0);
+        } else if (j == jmax - 1) {  // upper boundary (cc.f90:38-43)
+            v = (k < 3) ? 2.0 * colval(k, jmax - 2) - colval(k, jmax - 3) : (k == 3 ? ig : 1.0);
+        } else {
+            v = colval(k, j);
+        }
+        vals[k] = v;
+    }
+    for (int k = 0; k < E.nk; ++k) E.U[k][(size_t)j * pitch + i] = vals[k];
+}
+
+hipError_t launch_edge_i(const EdgeArgs &E, hipStream_t s)
+{
+    dim3 grid((E.jmax + 255) / 256, 2);
+    hipLaunchKernelGGL(edge_i_kernel, grid, dim3(256), 0, s, E);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Operator-level kernels for the parity API (one thread per point).
+// ---------------------------------------------------------------------------
+__global__ void deriv_kernel(const double *const *in, double *const *out, int nk, int imax, int jmax, int pitch,
+                             int ydir, const double *met, double inv_h)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= imax) return;
+    for (int k = 0; k < nk; ++k) {
+        const double *u = in[k];
+        double d;
+        if (!ydir) {
+            const double *row = u + (size_t)j * pitch;
+            d = drp_deriv<double>([row](int q) { return row[q]; }, i, imax, inv_h) * met[i];
+        } else {
+            const double *col = u + i;
+            d = drp_deriv<double>([col, pitch](int q) { return col[(size_t)q * pitch]; }, j, jmax, inv_h) * met[j];
+        }
+        out[k][(size_t)j * pitch + i] = d;
+    }
+}
+
+hipError_t launch_deriv(const double *const *in, double *const *out, int nk, int imax, int jmax, int pitch, bool ydir,
+                        const double *met, double inv_h, hipStream_t s)
+{
+    dim3 grid((imax + 127) / 128, jmax);
+    hipLaunchKernelGGL(deriv_kernel, grid, dim3(128), 0, s, in, out, nk, imax, jmax, pitch, (int)ydir, met, inv_h);
+    return hipGetLastError();
+}
+
+__global__ void transport_kernel(const double *T, double *MI, double *K1, size_t n, Transport<double> tc)
+{
+    const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double mu, kk;
+    transport_eval<double>(tc, T[q], mu, kk);
+    MI[q] = mu;
+    K1[q] = kk;
+}
+
+hipError_t launch_transport(const double *T, double *MI, double *K1, size_t n, const Transport<double> &tc,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(transport_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, T, MI, K1, n, tc);
+    return hipGetLastError();
+}
+
+// rk_steep on nk planes (rk.f90:160-167)
+__global__ void rk_steep_kernel(double *const *U, const double *const *dudt, double *const *W, int nk, int imax,
+                                int jmax, int pitch, double alpha, double beta, double dt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < 1 || i > imax - 2 || j < 1 || j > jmax - 2) return;
+    const size_t g = (size_t)j * pitch + i;
+    for (int k = 0; k < nk; ++k) {
+        const double wn = alpha * W[k][g] + dt * dudt[k][g];
+        W[k][g] = wn;
+        U[k][g] = U[k][g] + beta * wn;
+    }
+}
+
+hipError_t launch_rk_steep(double *const *U, const double *const *dudt, double *const *W, int nk, int imax, int jmax,
+                           int pitch, double alpha, double beta, double dt, hipStream_t s)
+{
+    dim3 grid((imax + 127) / 128, jmax);
+    hipLaunchKernelGGL(rk_steep_kernel, grid, dim3(128), 0, s, U, dudt, W, nk, imax, jmax, pitch, alpha, beta, dt);
+    return hipGetLastError();
+}
+
+// bufferxy (rk.f90:77-151)
+__global__ void buffer_kernel(double *const *U, const double *const *base, int nk, int imax, int jmax, int pitch,
+                              int D, const double *sigx, const double *sigy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= imax) return;
+    const bool ybuf = j < D || j >= jmax - D, xbuf = i < D || i >= imax - D;
+    if (!ybuf && !xbuf) return;
+    const size_t g = (size_t)j * pitch + i;
+    for (int k = 0; k < nk; ++k) {
+        double un = U[k][g];
+        const double ub = base[k][g];
+        if (ybuf) un = ub + sigy[j] * (un - ub);
+        if (xbuf) un = ub + sigx[i] * (un - ub);
+        U[k][g] = un;
+    }
+}
+
+hipError_t launch_buffer(double *const *U, const double *const *base, int nk, int imax, int jmax, int pitch, int D,
+                         const double *sigx, const double *sigy, hipStream_t s)
+{
+    dim3 grid((imax + 127) / 128, jmax);
+    hipLaunchKernelGGL(buffer_kernel, grid, dim3(128), 0, s, U, base, nk, imax, jmax, pitch, D, sigx, sigy);
+    return hipGetLastError();
+}
+
+// acousticfont (rk.f90:186-191), separable form amp * gx(i) * gy(j)
+__global__ void source_kernel(double *s_out, int imax, int jmax, double amp, const double *gx, const double *gy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= imax) return;
+    s_out[(size_t)j * imax + i] = amp * (gx[i] * gy[j]);
+}
+
+hipError_t launch_source(double *s_out, int imax, int jmax, double amp, const double *gx, const double *gy,
+                         hipStream_t s)
+{
+    dim3 grid((imax + 127) / 128, jmax);
+    hipLaunchKernelGGL(source_kernel, grid, dim3(128), 0, s, s_out, imax, jmax, amp, gx, gy);
+    return hipGetLastError();
+}
+
+// probes (euler.f90:156-161): gather nk values at npts points
+__global__ void probe_kernel(const double *const *U, int nk, int pitch, int npts, const int *ij, double *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts * nk) return;
+    const int pnt = t / nk, k = t - pnt * nk;
+    const int i = ij[2 * pnt] - 1, j = ij[2 * pnt + 1] - 1;
+    out[t] = U[k][(size_t)j * pitch + i];
+}
+
+hipError_t launch_probe(const double *const *U, int nk, int pitch, int npts, const int *ij, double *out,
+                        hipStream_t s)
+{
+    const int n = npts * nk;
+    hipLaunchKernelGGL(probe_kernel, dim3((n + 255) / 256), dim3(256), 0, s, U, nk, pitch, npts, ij, out);
+    return hipGetLastError();
+}
+
+// non-finite scan: flag = 1 if any value is NaN/Inf
+__global__ void finite_kernel(const double *const *U, int nk, int imax, int jmax, int pitch, int *flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= imax) return;
+    for (int k = 0; k < nk; ++k) {
+        const double v = U[k][(size_t)j * pitch + i];
+        if (!isfinite(v)) {
+            atomicOr(flag, 1);
+            return;
+        }
+    }
+}
+
+hipError_t launch_finite(const double *const *U, int nk, int imax, int jmax, int pitch, int *flag, hipStream_t s)
+{
+    dim3 grid((imax + 127) / 128, jmax);
+    hipLaunchKernelGGL(finite_kernel, grid, dim3(128), 0, s, U, nk, imax, jmax, pitch, flag);
+    return hipGetLastError();
+}
+
+// fill kernel (graph-safe zeroing)
+__global__ void fill_kernel(double *p, size_t n, double v)
+{
+    const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace nsamd

@@ -1,0 +1,189 @@
+"""Python mirror of the reference's hot-path interface over libnsamd.so.
+
+`Solver` owns one device context (what `module global` holds in the
+reference) and exposes the reference routines under their own names:
+
+    rk_euler(iter)                 rk.f90:12      one LDDRK6 step
+    run(iter0, nsteps)             euler.f90:123  the time loop
+    rhs_euler(U, ct, iter)         rhs.f90:10
+    deropm(U) / deropn(U)          diff.f90:6 / :127 (+ metrics)
+    molecular_mi_k(T)              coefficients.f90:27-59
+    rk_steep(a, b, U, dudt, w)     rk.f90:153
+    bufferxy(U)                    rk.f90:77
+    lddfilterx(U) / lddfiltery(U)  filtering.f90:9 / :98
+    ccpml(U)                       cc.f90:6
+    acousticfont(ct, iter)         rk.f90:171
+
+Host arrays follow the reference layout U(1:nk,1:imax,1:jmax), i.e. numpy
+shape (jmax, imax, nk), C-contiguous, float64.  Errors raise NSError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .config import Case
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+class Solver:
+    def __init__(self, case: Case, device: int = 0, precision: int = _lib.NS_PREC_FP64):
+        self.case = case
+        self.lib = _lib.load()
+        p = _lib.NSParams()
+        _lib.check(self.lib.ns_default_params(C.byref(p)))
+        tc = case.transport()
+        p.imax, p.jmax, p.nk, p.D = case.imax, case.jmax, case.nk, case.D
+        p.dm, p.dn, p.dt, p.sigmaf = case.dm, case.dn, case.dt, case.sigmaf
+        p.gamma, p.Re, p.Pr, p.Pe = case.gamma, case.Re, case.Pr, case.Pe
+        p.alpha_cond, p.Tref = case.alpha_cond, case.Tref
+        for key in ("A_mi", "B_mi", "C_mi", "D_mi", "MI0", "A_k", "B_k", "C_k", "D_k", "K0"):
+            setattr(p, key, tc[key])
+        p.amplitude, p.wf, p.r0_acous = case.amplitude, case.wf, case.r0_acous
+        p.m_pt_acous, p.n_pt_acous = case.m_pt_acous, case.n_pt_acous
+        p.precision = precision
+        flags = 0
+        if case.inviscid:
+            flags |= _lib.NS_FLAG_INVISCID
+        if case.metric_everywhere:
+            flags |= _lib.NS_FLAG_METRIC_EVERYWHERE
+        p.flags = flags
+        p.device = device
+        self.params = p
+        ctx = C.c_void_p()
+        _lib.check(self.lib.ns_create(C.byref(p), C.byref(ctx)))
+        self.ctx = ctx
+        self.shape = (case.jmax, case.imax, case.nk)
+        c = case
+        self._prof = [_f64(x) for x in (c.meshx, c.meshy, c.sigmax, c.sigmay, c.m, c.n, c.rhob, c.psib)]
+        _lib.check(self.lib.ns_set_profiles(self.ctx, *[_lib.dptr(x) for x in self._prof]))
+        self.set_base(c.Ub)
+        self.put_state(c.U0)
+
+    # lifetime ---------------------------------------------------------------
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.ns_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # state --------------------------------------------------------------------
+    def set_base(self, Ub):
+        Ub = _f64(Ub, self.shape)
+        _lib.check(self.lib.ns_set_base(self.ctx, _lib.dptr(Ub)))
+
+    def put_state(self, U):
+        U = _f64(U, self.shape)
+        _lib.check(self.lib.ns_put_state(self.ctx, _lib.dptr(U)))
+
+    def get_state(self) -> np.ndarray:
+        U = np.empty(self.shape)
+        _lib.check(self.lib.ns_get_state(self.ctx, _lib.dptr(U)))
+        return U
+
+    def probe(self, points) -> np.ndarray:
+        """Values at 1-based (i, j) points: shape (npts, nk) (euler.f90:156-161)."""
+        ij = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 2))
+        out = np.empty((ij.shape[0], self.case.nk))
+        _lib.check(self.lib.ns_probe(self.ctx, ij.shape[0], ij.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     _lib.dptr(out)))
+        return out
+
+    def check_finite(self):
+        _lib.check(self.lib.ns_check_finite(self.ctx))
+
+    def synchronize(self):
+        _lib.check(self.lib.ns_synchronize(self.ctx))
+
+    # time advance -------------------------------------------------------------
+    def rk_euler(self, iter: int):
+        _lib.check(self.lib.ns_rk_euler(self.ctx, int(iter)))
+
+    def run(self, iter0: int, nsteps: int):
+        _lib.check(self.lib.ns_rk_steps(self.ctx, int(iter0), int(nsteps)))
+
+    # operator mirrors ---------------------------------------------------------
+    def deropm(self, U) -> np.ndarray:
+        U = _f64(U)
+        nk = U.shape[-1] if U.ndim == 3 else 1
+        out = np.empty_like(U)
+        _lib.check(self.lib.ns_deropm(self.ctx, nk, _lib.dptr(out), _lib.dptr(U)))
+        return out
+
+    def deropn(self, U) -> np.ndarray:
+        U = _f64(U)
+        nk = U.shape[-1] if U.ndim == 3 else 1
+        out = np.empty_like(U)
+        _lib.check(self.lib.ns_deropn(self.ctx, nk, _lib.dptr(out), _lib.dptr(U)))
+        return out
+
+    def molecular_mi_k(self, T):
+        T = _f64(T, (self.case.jmax, self.case.imax))
+        mi, kk = np.empty_like(T), np.empty_like(T)
+        _lib.check(self.lib.ns_transport(self.ctx, _lib.dptr(T), _lib.dptr(mi), _lib.dptr(kk)))
+        return mi, kk
+
+    def rhs_euler(self, U, ct: float = 0.0, iter: int = 0) -> np.ndarray:
+        U = _f64(U, self.shape)
+        out = np.empty_like(U)
+        _lib.check(self.lib.ns_rhs(self.ctx, _lib.dptr(out), _lib.dptr(U), float(ct), int(iter)))
+        return out
+
+    def rk_steep(self, alphat: float, betat: float, U, dudt, w):
+        U = _f64(U, self.shape).copy()
+        w = _f64(w, self.shape).copy()
+        dudt = _f64(dudt, self.shape)
+        _lib.check(self.lib.ns_rk_steep(self.ctx, float(alphat), float(betat), _lib.dptr(U),
+                                        _lib.dptr(dudt), _lib.dptr(w)))
+        return U, w
+
+    def _inplace(self, fn, U):
+        U = _f64(U, self.shape).copy()
+        _lib.check(fn(self.ctx, _lib.dptr(U)))
+        return U
+
+    def bufferxy(self, U):
+        return self._inplace(self.lib.ns_bufferxy, U)
+
+    def lddfilterx(self, U):
+        return self._inplace(self.lib.ns_filter_x, U)
+
+    def lddfiltery(self, U):
+        return self._inplace(self.lib.ns_filter_y, U)
+
+    def ccpml(self, U):
+        return self._inplace(self.lib.ns_ccpml, U)
+
+    def acousticfont(self, ct: float, iter: int) -> np.ndarray:
+        s = np.empty((self.case.jmax, self.case.imax))
+        _lib.check(self.lib.ns_acousticfont(self.ctx, float(ct), int(iter), _lib.dptr(s)))
+        return s
+
+    # measurement --------------------------------------------------------------
+    def enable_timing(self, on: bool = True):
+        _lib.check(self.lib.ns_enable_timing(self.ctx, int(on)))
+
+    def kernel_time(self, name: str):
+        avg = C.c_double()
+        cnt = C.c_int32()
+        _lib.check(self.lib.ns_kernel_time(self.ctx, name.encode(), C.byref(avg), C.byref(cnt)))
+        return avg.value, cnt.value
