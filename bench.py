@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp64", "fp32pert"], default="fp64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -79,8 +80,11 @@ def main():
     from navierstokes_amd.solver import Solver
 
     case, desc = make_case(args)
+    prec = 1 if args.precision == "fp32pert" else 0
+    if prec:
+        desc = desc.replace("fp64", "fp32-perturbation")
     print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
-    s = Solver(case)
+    s = Solver(case, precision=prec)
     s.run(0, args.warmup)
     s.synchronize()
     print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
@@ -99,7 +103,7 @@ def main():
     s.enable_timing(False)
     s.close()
 
-    nk, elem = case.nk, 8
+    nk, elem = case.nk, (4 if prec else 8)
     B_step = alg_bytes_per_point_step(nk, elem)
     B_stage = B_step / 6.0  # per point per stage-kernel launch (average over the 6 stages)
     t_stage = kt["rhs"] * 1e-3
@@ -116,9 +120,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if prec else "f64",
         "data": "synthetic",
-        "config": {"workload": desc, "grid": [case.imax, case.jmax], "nk": nk, "precision": "fp64",
+        "config": {"workload": desc, "grid": [case.imax, case.jmax], "nk": nk, "precision": args.precision,
                    "viscous": not case.inviscid, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,

@@ -7,7 +7,7 @@
  * boundary that replaces them: a context (`ns_ctx`) holds, on one GPU, what
  * `global` holds on the host (scalars, 1-D profiles, base flow, state) and
  * every reference routine on the path has an entry point here.  The Fortran
- * shim (navierstokes_amd/fortran/ns_shim.f90) keeps the reference's module
+ * layer (navierstokes_amd/fortran/ns_capi.f90, ns_dropin.f90) keeps the reference's module
  * and procedure names and calls these through ISO_C_BINDING; the Python layer
  * (navierstokes_amd/solver.py) calls them through ctypes.
  *
@@ -46,7 +46,8 @@ extern "C" {
 #define NS_PREC_FP32_PERT 1 /* FP32 perturbation about the FP64 base flow */
 
 /* flags */
-#define NS_FLAG_INVISCID 1u           /* drop the 1/Re, 1/(Re Pr), 1/Pe terms */
+#define NS_FLAG_INVISCID 1u           /* drop the 1/Re and 1/(Re Pr) terms (Re -> inf);
+                                         the species diffusion 1/Pe stays */
 #define NS_FLAG_METRIC_EVERYWHERE 2u  /* apply meshx/meshy at every point (stretched
                                          grids); default = reference region tests */
 

@@ -21,8 +21,11 @@
 
 namespace nsamd {
 hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, hipStream_t s);
+hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, hipStream_t s);
 hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s);
-hipError_t launch_edge_i(const EdgeArgs &E, hipStream_t s);
+hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s);
+hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s);
+hipError_t launch_edge_i(const EdgeArgs<float> &E, hipStream_t s);
 hipError_t launch_deriv(const double *const *in, double *const *out, int nk, int imax, int jmax, int pitch, bool ydir,
                         const double *met, double inv_h, hipStream_t s);
 hipError_t launch_transport(const double *T, double *MI, double *K1, size_t n, const Transport<double> &tc,
@@ -33,9 +36,10 @@ hipError_t launch_buffer(double *const *U, const double *const *base, int nk, in
                          const double *sigx, const double *sigy, hipStream_t s);
 hipError_t launch_source(double *s_out, int imax, int jmax, double amp, const double *gx, const double *gy,
                          hipStream_t s);
-hipError_t launch_probe(const double *const *U, int nk, int pitch, int npts, const int *ij, double *out,
-                        hipStream_t s);
-hipError_t launch_finite(const double *const *U, int nk, int imax, int jmax, int pitch, int *flag, hipStream_t s);
+template <typename S>
+hipError_t launch_probe(const S *const *U, int nk, int pitch, int npts, const int *ij, double *out, hipStream_t s);
+template <typename S>
+hipError_t launch_finite(const S *const *U, int nk, int imax, int jmax, int pitch, int *flag, hipStream_t s);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
 }  // namespace nsamd
 
@@ -73,9 +77,14 @@ struct ns_ctx {
     size_t plane = 0;  // elements per plane
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    double *S[3] = {nullptr, nullptr, nullptr};
-    double *Wr = nullptr;     // LDDRK register
-    double *base = nullptr;   // Ub planes
+    bool pert = false;        // FP32 perturbation form
+    size_t esz = 8;           // bytes per state element
+    void *S[3] = {nullptr, nullptr, nullptr};  // state buffers (FP64 U or FP32 q)
+    void *Wr = nullptr;       // LDDRK register (same precision as the state)
+    double *base = nullptr;   // Ub planes (FP64)
+    double *prow = nullptr;   // perturbation form: per-row base data (PertRows)
+    std::vector<double> ubh;  // perturbation form: host copy of Ub(k, j)
+    RowEdge re{};             // ccpml row constants
     double *work = nullptr;   // operator-API scratch (2 x 8 planes)
     double *prof = nullptr;   // 1-D profiles
     double *metx = nullptr, *mety = nullptr, *sigx = nullptr, *sigy = nullptr;
@@ -94,6 +103,37 @@ struct ns_ctx {
 namespace {
 
 double *plane_ptr(double *buf, const ns_ctx *c, int k) { return buf + (size_t)k * c->plane; }
+template <typename S>
+S *splane(void *buf, const ns_ctx *c, int k) { return static_cast<S *>(buf) + (size_t)k * c->plane; }
+
+// ccpml row constants (cc.f90:38-59) for the FP64 form
+RowEdge fp64_row_edge(const ns_params &P)
+{
+    RowEdge re{};
+    for (int k = 0; k < NKMAX; ++k) re.bot_res[k] = re.top_res[k] = 0.0;
+    re.bot_val[3] = re.top_val[3] = 1.0 / P.gamma;
+    re.bot_val[4] = 0.0;
+    re.top_val[4] = 1.0;
+    return re;
+}
+
+// per-row base block: ub[nk], dub[nk], tb, dtb, ycorr[nk] (jmax each)
+size_t prow_count(const ns_params &P) { return (size_t)(3 * P.nk + 2) * P.jmax; }
+
+PertRows pert_rows(const ns_ctx *c)
+{
+    PertRows pr{};
+    const size_t J = (size_t)c->P.jmax;
+    const int nk = c->P.nk;
+    for (int k = 0; k < NKMAX; ++k) {
+        pr.ub[k] = k < nk ? c->prow + k * J : nullptr;
+        pr.dub[k] = k < nk ? c->prow + (nk + k) * J : nullptr;
+        pr.ycorr[k] = k < nk ? c->prow + (2 * nk + 2 + k) * J : nullptr;
+    }
+    pr.tb = c->prow + 2 * nk * J;
+    pr.dtb = c->prow + (2 * nk + 1) * J;
+    return pr;
+}
 
 // 1-D profile block: three arrays along i, five along j, one allocation
 struct ProfileLayout {
@@ -156,18 +196,20 @@ int filter_warmup(double sigma)
     return -1;
 }
 
-StageArgs<double> stage_args(ns_ctx *c, int irk, double *Uin, double *Uout, double amp)
+template <typename S>
+StageArgs<S> stage_args(ns_ctx *c, int irk, S *Uin, S *Uout, double amp, bool pert)
 {
     const ns_params &P = c->P;
-    StageArgs<double> A{};
+    StageArgs<S> A{};
     for (int k = 0; k < NKMAX; ++k) {
         const bool on = k < P.nk;
-        A.U[k] = on ? plane_ptr(Uin, c, k) : nullptr;
-        A.Uout[k] = on ? plane_ptr(Uout, c, k) : nullptr;
-        A.W[k] = on ? plane_ptr(c->Wr, c, k) : nullptr;
-        A.base[k] = on ? plane_ptr(c->base, c, k) : nullptr;
+        A.U[k] = on ? Uin + (size_t)k * c->plane : nullptr;
+        A.Uout[k] = on ? Uout + (size_t)k * c->plane : nullptr;
+        A.W[k] = on ? splane<S>(c->Wr, c, k) : nullptr;
+        A.base[k] = (on && !pert) ? reinterpret_cast<const S *>(plane_ptr(c->base, c, k)) : nullptr;
         A.dudt[k] = nullptr;
     }
+    if (pert) A.pr = pert_rows(c);
     A.metx = c->metx;
     A.mety = c->mety;
     A.sigx = c->sigx;
@@ -196,30 +238,39 @@ StageArgs<double> stage_args(ns_ctx *c, int irk, double *Uin, double *Uout, doub
     return A;
 }
 
-FilterArgs<double> filter_args(ns_ctx *c, bool xdir, double *in, double *out, bool edge_bc)
+template <typename S>
+FilterArgs<S> filter_args(ns_ctx *c, bool xdir, S *in, S *out, bool edge_bc, bool pert)
 {
     const ns_params &P = c->P;
-    FilterArgs<double> F{};
+    FilterArgs<S> F{};
+    const PertRows pr = pert ? pert_rows(c) : PertRows{};
     for (int k = 0; k < NKMAX; ++k) {
-        F.in[k] = k < P.nk ? plane_ptr(in, c, k) : nullptr;
-        F.out[k] = k < P.nk ? plane_ptr(out, c, k) : nullptr;
+        F.in[k] = k < P.nk ? in + (size_t)k * c->plane : nullptr;
+        F.out[k] = k < P.nk ? out + (size_t)k * c->plane : nullptr;
+        F.corr[k] = (pert && !xdir && k < P.nk) ? pr.ycorr[k] : nullptr;
     }
+    F.re = pert ? c->re : fp64_row_edge(P);
     F.n = xdir ? P.imax : P.jmax;
     F.m = xdir ? P.jmax : P.imax;
     F.pitch = c->pitch;
     F.L = c->filt_L;
     F.W = c->filt_W;
     F.sigma = P.sigmaf;
-    F.gamma = P.gamma;
     F.nk = P.nk;
     F.edge_bc = edge_bc;
     return F;
 }
 
-EdgeArgs edge_args(ns_ctx *c, double *U)
+template <typename S>
+EdgeArgs<S> edge_args(ns_ctx *c, S *U, bool pert)
 {
-    EdgeArgs E{};
-    for (int k = 0; k < NKMAX; ++k) E.U[k] = k < c->P.nk ? plane_ptr(U, c, k) : nullptr;
+    EdgeArgs<S> E{};
+    const PertRows pr = pert ? pert_rows(c) : PertRows{};
+    for (int k = 0; k < NKMAX; ++k) {
+        E.U[k] = k < c->P.nk ? U + (size_t)k * c->plane : nullptr;
+        E.ub[k] = (pert && k < c->P.nk) ? pr.ub[k] : nullptr;
+    }
+    E.re = pert ? c->re : fp64_row_edge(c->P);
     E.rhob = c->rhob;
     E.psib = c->P.nk >= 5 ? c->psib : nullptr;
     E.imax = c->P.imax;
@@ -237,23 +288,26 @@ double stage_amp(const ns_params &P, int irk, int iter)
 }
 
 // one LDDRK6 step on the device, rk.f90:52-73
-int enqueue_step(ns_ctx *c, int iter, bool timed)
+template <typename S>
+int enqueue_step_t(ns_ctx *c, int iter, bool timed)
 {
     const ns_params &P = c->P;
     const bool visc = !(P.flags & NS_FLAG_INVISCID);
+    const bool pert = c->pert;
+    S *S0 = static_cast<S *>(c->S[0]), *S1 = static_cast<S *>(c->S[1]), *S2 = static_cast<S *>(c->S[2]);
     for (int irk = 0; irk < 6; ++irk) {
         auto tick = [&](int slot) {
             if (timed) hipEventRecord(c->ev[slot], c->stream);
         };
-        StageArgs<double> A = stage_args(c, irk, c->S[0], c->S[1], stage_amp(P, irk, iter));
+        StageArgs<S> A = stage_args<S>(c, irk, S0, S1, stage_amp(P, irk, iter), pert);
         tick(0);
         HIPCHK(launch_stage(A, P.nk, visc, false, c->stream));
         tick(1);
-        HIPCHK(launch_filter(filter_args(c, true, c->S[1], c->S[2], false), true, c->stream));
+        HIPCHK(launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, c->stream));
         tick(2);
-        HIPCHK(launch_filter(filter_args(c, false, c->S[2], c->S[0], true), false, c->stream));
+        HIPCHK(launch_filter(filter_args<S>(c, false, S2, S0, true, pert), false, c->stream));
         tick(3);
-        HIPCHK(launch_edge_i(edge_args(c, c->S[0]), c->stream));
+        HIPCHK(launch_edge_i(edge_args<S>(c, S0, pert), c->stream));
         tick(4);
         if (timed) {
             HIPCHK(hipEventSynchronize(c->ev[4]));
@@ -266,6 +320,11 @@ int enqueue_step(ns_ctx *c, int iter, bool timed)
         }
     }
     return NS_OK;
+}
+
+int enqueue_step(ns_ctx *c, int iter, bool timed)
+{
+    return c->pert ? enqueue_step_t<float>(c, iter, timed) : enqueue_step_t<double>(c, iter, timed);
 }
 
 int check_ready(ns_ctx *c)
@@ -303,10 +362,125 @@ int download_aos(ns_ctx *c, const double *src, double *U, int nk)
     return NS_OK;
 }
 
-int set_ptr_table(ns_ctx *c, int slot, double *buf, int nk)
+int set_ptr_table(ns_ctx *c, int slot, void *buf, int nk, size_t esz = sizeof(double));
+
+// state <-> host AoS; the perturbation form stores q = U - Ub(k,j) in FP32
+int upload_state(ns_ctx *c, const double *U)
 {
-    double *h[8];
-    for (int k = 0; k < 8; ++k) h[k] = k < nk ? plane_ptr(buf, c, k) : nullptr;
+    if (!c->pert) return upload_aos(c, static_cast<double *>(c->S[0]), U, c->P.nk);
+    const ns_params &P = c->P;
+    const int nk = P.nk;
+    const size_t J = (size_t)P.jmax;
+    std::vector<float> tmp((size_t)nk * c->plane, 0.f);
+    for (int j = 0; j < P.jmax; ++j)
+        for (int i = 0; i < P.imax; ++i)
+            for (int k = 0; k < nk; ++k)
+                tmp[(size_t)k * c->plane + (size_t)j * c->pitch + i] =
+                    (float)(U[k + (size_t)nk * (i + (size_t)P.imax * j)] - c->ubh[k * J + j]);
+    HIPCHK(hipMemcpyAsync(c->S[0], tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int download_state(ns_ctx *c, double *U)
+{
+    if (!c->pert) return download_aos(c, static_cast<const double *>(c->S[0]), U, c->P.nk);
+    const ns_params &P = c->P;
+    const int nk = P.nk;
+    const size_t J = (size_t)P.jmax;
+    std::vector<float> tmp((size_t)nk * c->plane);
+    HIPCHK(hipMemcpyAsync(tmp.data(), c->S[0], tmp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < P.jmax; ++j)
+        for (int i = 0; i < P.imax; ++i)
+            for (int k = 0; k < nk; ++k)
+                U[k + (size_t)nk * (i + (size_t)P.imax * j)] =
+                    c->ubh[k * J + j] + (double)tmp[(size_t)k * c->plane + (size_t)j * c->pitch + i];
+    return NS_OK;
+}
+
+// column i=0 of nplanes device planes -> host rows (jmax each)
+int extract_column(ns_ctx *c, const double *planes, int nplanes, double *rows)
+{
+    const ns_params &P = c->P;
+    for (int k = 0; k < nplanes; ++k)
+        HIPCHK(hipMemcpy2DAsync(rows + (size_t)k * P.jmax, sizeof(double), planes + (size_t)k * c->plane,
+                                (size_t)c->pitch * sizeof(double), sizeof(double), P.jmax, hipMemcpyDeviceToHost,
+                                c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+// Per-row base data of the perturbation form, computed with the FP64 kernels:
+// d Ub/dy and d Tb/dy (DRP + metric) and Ly(Ub) - Ub (the in-place y filter
+// acting on the base, which includes the float32 boundary literals' drift).
+int setup_perturbation(ns_ctx *c, const double *Ub)
+{
+    const ns_params &P = c->P;
+    const int I = P.imax, J = P.jmax, nk = P.nk;
+    c->ubh.assign((size_t)nk * J, 0.0);
+    for (int j = 0; j < J; ++j)
+        for (int k = 0; k < nk; ++k) {
+            const double v = Ub[k + (size_t)nk * ((size_t)I * j)];
+            for (int i = 1; i < I; ++i)
+                if (Ub[k + (size_t)nk * (i + (size_t)I * j)] != v)
+                    return fail(NS_ERR_PARAM,
+                                "the FP32 perturbation form needs a base flow constant along each row "
+                                "(Ub(%d,%d,%d) differs from Ub(%d,1,%d))", k + 1, i + 1, j + 1, k + 1, j + 1);
+            c->ubh[(size_t)k * J + j] = v;
+        }
+    std::vector<double> rows(prow_count(P), 0.0);
+    const PertRows off = pert_rows(c);
+    auto at = [&](const double *dp) { return rows.data() + (dp - c->prow); };
+    std::copy(c->ubh.begin(), c->ubh.end(), at(off.ub[0]));
+    double *tb = at(off.tb);
+    for (int j = 0; j < J; ++j) tb[j] = P.gamma * c->ubh[3 * (size_t)J + j] / c->ubh[j];
+    double *outb = c->work + 8 * c->plane;
+    int rc;
+    // d Ub / dy
+    if ((rc = set_ptr_table(c, 0, c->base, nk)) || (rc = set_ptr_table(c, 1, outb, nk))) return rc;
+    HIPCHK(launch_deriv((const double *const *)c->dptrs, (double *const *)(c->dptrs + 8), nk, I, J, c->pitch, true,
+                        c->mety, 1.0 / P.dn, c->stream));
+    if ((rc = extract_column(c, outb, nk, at(off.dub[0])))) return rc;
+    // d Tb / dy
+    {
+        std::vector<double> tp(c->plane, 0.0);
+        for (int j = 0; j < J; ++j)
+            for (int i = 0; i < I; ++i) tp[(size_t)j * c->pitch + i] = tb[j];
+        HIPCHK(hipMemcpyAsync(c->work, tp.data(), tp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        if ((rc = set_ptr_table(c, 0, c->work, 1))) return rc;
+        HIPCHK(launch_deriv((const double *const *)c->dptrs, (double *const *)(c->dptrs + 8), 1, I, J, c->pitch,
+                            true, c->mety, 1.0 / P.dn, c->stream));
+        if ((rc = extract_column(c, outb, 1, at(off.dtb)))) return rc;
+    }
+    // Ly(Ub) - Ub
+    HIPCHK(launch_filter(filter_args<double>(c, false, c->base, outb, false, false), false, c->stream));
+    double *yc = at(off.ycorr[0]);
+    if ((rc = extract_column(c, outb, nk, yc))) return rc;
+    for (size_t q = 0; q < (size_t)nk * J; ++q) yc[q] -= c->ubh[q];
+    HIPCHK(hipMemcpyAsync(c->prow, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // ccpml rows in perturbation form (cc.f90:38-59 minus the base rows)
+    RowEdge re = fp64_row_edge(P);
+    auto ub = [&](int k, int j) { return c->ubh[(size_t)k * J + j]; };
+    for (int k = 0; k < 3; ++k) {
+        re.bot_res[k] = 2.0 * ub(k, 1) - ub(k, 2) - ub(k, 0);
+        re.top_res[k] = 2.0 * ub(k, J - 2) - ub(k, J - 3) - ub(k, J - 1);
+    }
+    re.bot_val[3] = 1.0 / P.gamma - ub(3, 0);
+    re.top_val[3] = 1.0 / P.gamma - ub(3, J - 1);
+    if (nk >= 5) {
+        re.bot_val[4] = 0.0 - ub(4, 0);
+        re.top_val[4] = 1.0 - ub(4, J - 1);
+    }
+    c->re = re;
+    return NS_OK;
+}
+
+int set_ptr_table(ns_ctx *c, int slot, void *buf, int nk, size_t esz)
+{
+    void *h[8];
+    for (int k = 0; k < 8; ++k) h[k] = k < nk ? static_cast<char *>(buf) + (size_t)k * c->plane * esz : nullptr;
     HIPCHK(hipMemcpyAsync(c->dptrs + 8 * slot, h, sizeof h, hipMemcpyHostToDevice, c->stream));
     return NS_OK;
 }
@@ -314,7 +488,8 @@ int set_ptr_table(ns_ctx *c, int slot, double *buf, int nk)
 void release(ns_ctx *c)
 {
     if (c->graph) hipGraphExecDestroy(c->graph);
-    for (double *p : {c->S[0], c->S[1], c->S[2], c->Wr, c->base, c->work, c->prof})
+    for (void *p : {c->S[0], c->S[1], c->S[2], c->Wr, (void *)c->base, (void *)c->work, (void *)c->prof,
+                    (void *)c->prow})
         if (p) hipFree(p);
     if (c->dptrs) hipFree(c->dptrs);
     if (c->dflag) hipFree(c->dflag);
@@ -384,8 +559,8 @@ int32_t ns_create(const ns_params *p, ns_ctx **out)
         return fail(NS_ERR_PARAM, "buffer width D=%d must satisfy 0 <= 2D <= min(imax, jmax)", P.D);
     if (!(P.dm > 0) || !(P.dn > 0) || !(P.dt > 0)) return fail(NS_ERR_PARAM, "dm, dn, dt must be positive");
     if (!(P.sigmaf >= 0.0 && P.sigmaf <= 1.0)) return fail(NS_ERR_PARAM, "sigmaf must lie in [0,1]");
-    if (P.precision != NS_PREC_FP64)
-        return fail(NS_ERR_PARAM, "precision mode %d is not available in this build", P.precision);
+    if (P.precision != NS_PREC_FP64 && P.precision != NS_PREC_FP32_PERT)
+        return fail(NS_ERR_PARAM, "unknown precision mode %d", P.precision);
     if (!(P.gamma > 1.0) || !(P.Re > 0) || !(P.Pr > 0) || !(P.Pe > 0) || !(P.Tref > 0) || !(P.MI0 > 0) || !(P.K0 > 0))
         return fail(NS_ERR_PARAM, "gamma, Re, Pr, Pe, Tref, MI0, K0 must be positive (gamma > 1)");
     const int W = filter_warmup(P.sigmaf);
@@ -393,6 +568,8 @@ int32_t ns_create(const ns_params *p, ns_ctx **out)
 
     ns_ctx *c = new ns_ctx();
     c->P = P;
+    c->pert = P.precision == NS_PREC_FP32_PERT;
+    c->esz = c->pert ? sizeof(float) : sizeof(double);
     c->filt_W = W;
     c->filt_L = 256;
     c->pitch = (P.imax + 63) / 64 * 64;
@@ -412,13 +589,21 @@ int32_t ns_create(const ns_params *p, ns_ctx **out)
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
     c->own_stream = true;
-    const size_t bytes = (size_t)P.nk * c->plane * sizeof(double);
-    for (double **pp : {&c->S[0], &c->S[1], &c->S[2], &c->Wr, &c->base}) {
-        e = hipMalloc(pp, bytes);
-        if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e)));
-        e = hipMemset(*pp, 0, bytes);
+    const size_t sbytes = (size_t)P.nk * c->plane * c->esz;
+    for (void **pp : {&c->S[0], &c->S[1], &c->S[2], &c->Wr}) {
+        e = hipMalloc(pp, sbytes);
+        if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(%zu): %s", sbytes, hipGetErrorString(e)));
+        e = hipMemset(*pp, 0, sbytes);
         if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)));
     }
+    const size_t bbytes = (size_t)P.nk * c->plane * sizeof(double);
+    e = hipMalloc(&c->base, bbytes);
+    if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(%zu): %s", bbytes, hipGetErrorString(e)));
+    e = hipMemset(c->base, 0, bbytes);
+    if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)));
+    e = hipMalloc(&c->prow, prow_count(P) * sizeof(double));
+    if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(rows): %s", hipGetErrorString(e)));
+    c->re = fp64_row_edge(P);
     e = hipMalloc(&c->work, 16 * c->plane * sizeof(double));
     if (e != hipSuccess) return bail(fail(NS_ERR_NOMEM, "hipMalloc(work): %s", hipGetErrorString(e)));
     e = hipMalloc(&c->prof, profile_count(P) * sizeof(double));
@@ -524,14 +709,20 @@ int32_t ns_set_base(ns_ctx *c, const double *Ub)
 {
     if (!c || !Ub) return fail(NS_ERR_PARAM, "null argument");
     int rc = upload_aos(c, c->base, Ub, c->P.nk);
-    if (rc == NS_OK) c->have_base = true;
-    return rc;
+    if (rc) return rc;
+    if (c->pert) {
+        if (!c->have_profiles) return fail(NS_ERR_STATE, "the FP32 perturbation form needs ns_set_profiles first");
+        if ((rc = setup_perturbation(c, Ub))) return rc;
+    }
+    c->have_base = true;
+    return NS_OK;
 }
 
 int32_t ns_put_state(ns_ctx *c, const double *U)
 {
     if (!c || !U) return fail(NS_ERR_PARAM, "null argument");
-    int rc = upload_aos(c, c->S[0], U, c->P.nk);
+    if (c->pert && !c->have_base) return fail(NS_ERR_STATE, "the FP32 perturbation form needs ns_set_base first");
+    int rc = upload_state(c, U);
     if (rc == NS_OK) c->have_state = true;
     return rc;
 }
@@ -539,7 +730,7 @@ int32_t ns_put_state(ns_ctx *c, const double *U)
 int32_t ns_get_state(ns_ctx *c, double *U)
 {
     if (!c || !U) return fail(NS_ERR_PARAM, "null argument");
-    return download_aos(c, c->S[0], U, c->P.nk);
+    return download_state(c, U);
 }
 
 int32_t ns_probe(ns_ctx *c, int32_t npts, const int32_t *ij, double *out)
@@ -554,25 +745,36 @@ int32_t ns_probe(ns_ctx *c, int32_t npts, const int32_t *ij, double *out)
     double *dout = nullptr;
     HIPCHK(hipMalloc(&dij, sizeof(int) * 2 * npts));
     HIPCHK(hipMalloc(&dout, sizeof(double) * npts * nk));
-    int rc = set_ptr_table(c, 0, c->S[0], nk);
+    int rc = set_ptr_table(c, 0, c->S[0], nk, c->esz);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(dij, ij, sizeof(int) * 2 * npts, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_probe((const double *const *)c->dptrs, nk, c->pitch, npts, dij, dout, c->stream));
+    if (c->pert)
+        HIPCHK(launch_probe<float>((const float *const *)c->dptrs, nk, c->pitch, npts, dij, dout, c->stream));
+    else
+        HIPCHK(launch_probe<double>((const double *const *)c->dptrs, nk, c->pitch, npts, dij, dout, c->stream));
     HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * npts * nk, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     hipFree(dij);
     hipFree(dout);
+    if (c->pert)
+        for (int t = 0; t < npts; ++t)
+            for (int k = 0; k < nk; ++k) out[t * nk + k] += c->ubh[(size_t)k * c->P.jmax + (ij[2 * t + 1] - 1)];
     return NS_OK;
 }
 
 int32_t ns_check_finite(ns_ctx *c)
 {
     if (!c) return fail(NS_ERR_PARAM, "null context");
-    int rc = set_ptr_table(c, 0, c->S[0], c->P.nk);
+    int rc = set_ptr_table(c, 0, c->S[0], c->P.nk, c->esz);
     if (rc) return rc;
     int zero = 0, flag = 0;
     HIPCHK(hipMemcpyAsync(c->dflag, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_finite((const double *const *)c->dptrs, c->P.nk, c->P.imax, c->P.jmax, c->pitch, c->dflag, c->stream));
+    if (c->pert)
+        HIPCHK(launch_finite<float>((const float *const *)c->dptrs, c->P.nk, c->P.imax, c->P.jmax, c->pitch, c->dflag,
+                                    c->stream));
+    else
+        HIPCHK(launch_finite<double>((const double *const *)c->dptrs, c->P.nk, c->P.imax, c->P.jmax, c->pitch,
+                                     c->dflag, c->stream));
     HIPCHK(hipMemcpyAsync(&flag, c->dflag, sizeof flag, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (flag) return fail(NS_ERR_NONFINITE, "state holds NaN or Inf");
@@ -664,7 +866,7 @@ int32_t ns_rhs(ns_ctx *c, double *dUdt, const double *U, double ct, int32_t iter
     double *in = c->work, *outb = c->work + 8 * c->plane;
     if ((rc = upload_aos(c, in, U, P.nk))) return rc;
     double amp = P.amplitude == 0.0 ? 0.0 : P.amplitude * std::sin(P.wf * (((double)iter + ct) * P.dt));
-    StageArgs<double> A = stage_args(c, 0, in, outb, amp);
+    StageArgs<double> A = stage_args<double>(c, 0, in, outb, amp, false);
     for (int k = 0; k < P.nk; ++k) A.dudt[k] = plane_ptr(outb, c, k);
     HIPCHK(launch_stage(A, P.nk, !(P.flags & NS_FLAG_INVISCID), true, c->stream));
     return download_aos(c, outb, dUdt, P.nk);
@@ -708,7 +910,7 @@ static int filter_common(ns_ctx *c, double *U, bool xdir)
     double *in = c->work, *outb = c->work + 8 * c->plane;
     int rc;
     if ((rc = upload_aos(c, in, U, nk))) return rc;
-    HIPCHK(launch_filter(filter_args(c, xdir, in, outb, false), xdir, c->stream));
+    HIPCHK(launch_filter(filter_args<double>(c, xdir, in, outb, false, false), xdir, c->stream));
     return download_aos(c, outb, U, nk);
 }
 
@@ -727,10 +929,10 @@ int32_t ns_ccpml(ns_ctx *c, double *U)
     // rows j=1, jmax of interior columns come from the y-filter kernel's edge
     // epilogue (run here at sigma = 0, which leaves every line unchanged),
     // then columns i=1, imax with the corners from edge_i_kernel.
-    FilterArgs<double> F = filter_args(c, false, dU, c->work + 8 * c->plane, true);
+    FilterArgs<double> F = filter_args<double>(c, false, dU, c->work + 8 * c->plane, true, false);
     F.sigma = 0.0;
     HIPCHK(launch_filter(F, false, c->stream));
-    HIPCHK(launch_edge_i(edge_args(c, c->work + 8 * c->plane), c->stream));
+    HIPCHK(launch_edge_i(edge_args<double>(c, c->work + 8 * c->plane, false), c->stream));
     return download_aos(c, c->work + 8 * c->plane, U, nk);
 }
 

@@ -1,5 +1,12 @@
-// ns_kernels.h -- kernel argument blocks and launcher declarations for the
-// FP64 time-advance path (gfx950).
+// ns_kernels.h -- kernel argument blocks for the time-advance path (gfx950).
+//
+// Two precision modes share the kernels:
+//   FP64       S = R = double, the state is the reference's U itself;
+//   FP32-pert  S = R = float, the state is q = U - Ub(j): the perturbation
+//              about a base flow that is constant along each grid row (the
+//              reference's base flows depend on y only, init.f90:368-404).
+//              The base rows, their y-derivatives, the base temperature and
+//              the y filter's action on the base stay FP64 (per-row arrays).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,44 +17,65 @@ namespace nsamd {
 
 constexpr int NKMAX = 5;
 
+// Per-row base-flow data of the perturbation form (all FP64, length jmax).
+struct PertRows {
+    const double *ub[NKMAX];   // Ub(k, j)
+    const double *dub[NKMAX];  // d Ub / dy (k, j) including the metric factor
+    const double *tb;          // base temperature gamma*pb/rhob
+    const double *dtb;         // d Tb / dy including the metric factor
+    const double *ycorr[NKMAX];  // Ly(Ub) - Ub: the y filter acting on the base
+};
+
 // Arguments of the fused stage kernel (RHS + LDDRK update + buffer zone).
-template <typename R>
+template <typename S>
 struct StageArgs {
-    const R *U[NKMAX];     // filtered state at the start of the stage
-    R *Uout[NKMAX];        // updated + buffered state (input of the filters)
-    R *W[NKMAX];           // 2N-storage register, updated in place
-    const R *base[NKMAX];  // base flow Ub
-    R *dudt[NKMAX];        // RHS output (rhs-only mode)
-    const R *metx, *mety;  // effective metric factors per column / row
-    const R *sigx, *sigy;  // buffer-zone profiles
-    const R *gx, *gy;      // separable source profile
+    const S *U[NKMAX];     // state at the start of the stage (filtered)
+    S *Uout[NKMAX];        // updated + buffered state (input of the filters)
+    S *W[NKMAX];           // 2N-storage register, updated in place
+    const S *base[NKMAX];  // base flow Ub planes (FP64 mode)
+    S *dudt[NKMAX];        // RHS output (rhs-only mode)
+    const double *metx, *mety;  // effective metric factors per column / row
+    const double *sigx, *sigy;  // buffer-zone profiles
+    const double *gx, *gy;      // separable source profile
+    PertRows pr;                // perturbation form only
     int imax, jmax, pitch, D;
-    R inv_dm, inv_dn, dt, alpha, beta;
-    R gamma, inv_Re, gm1_Re, inv_RePr, inv_Pe, alpha_cond;
-    R amp_src;  // amplitude*sin(wf*(iter+c)*dt) of this stage (0: no source)
-    Transport<R> tc;
+    double inv_dm, inv_dn, dt, alpha, beta;
+    double gamma, inv_Re, gm1_Re, inv_RePr, inv_Pe, alpha_cond;
+    double amp_src;  // amplitude*sin(wf*(iter+c)*dt) of this stage (0: no source)
+    Transport<double> tc;
     int first, last;  // stage 1 (alpha=0: W not read), stage 6 (W not written)
 };
 
+// ccpml constants of the rows j=1 / j=jmax (cc.f90:38-59): components 0-2
+// extrapolate (plus a base residual in the perturbation form), 3-4 are fixed.
+struct RowEdge {
+    double bot_res[NKMAX], top_res[NKMAX];  // added to 2 q(2) - q(3)
+    double bot_val[NKMAX], top_val[NKMAX];  // fixed values (components 3, 4)
+};
+
 // Arguments of the in-place selective filter (look-back chunked sweep).
-template <typename R>
+template <typename S>
 struct FilterArgs {
-    const R *in[NKMAX];
-    R *out[NKMAX];
+    const S *in[NKMAX];
+    S *out[NKMAX];
+    const double *corr[NKMAX];  // y filter, perturbation form: Ly(Ub) - Ub per row
     int n;          // points along the filtered direction
     int m;          // points across (rows for x, columns for y)
     int pitch;
     int L;  // chunk length
     int W;  // warm-up (look-back) length
-    R sigma;
-    R gamma;
+    double sigma;
     int nk;
     int edge_bc;  // y filter only: write the j=1 / j=jmax rows of ccpml
+    RowEdge re;
 };
 
+template <typename S>
 struct EdgeArgs {
-    double *U[NKMAX];
+    S *U[NKMAX];
     const double *rhob, *psib;
+    const double *ub[NKMAX];  // perturbation form: base rows (else null)
+    RowEdge re;
     int imax, jmax, pitch, nk;
     double gamma;
 };

@@ -42,16 +42,18 @@ __device__ __forceinline__ Range tile_ranges(int t0, int t1, int n)
     return r;
 }
 
-template <typename R, int NK, bool VISC, int TY, bool RHS_ONLY>
-__global__ __launch_bounds__(256) void stage_kernel(StageArgs<R> A)
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT>
+__global__ __launch_bounds__(256) void stage_kernel(StageArgs<S> A)
 {
     constexpr int TX = STX;
     constexpr int WX = TX + 12, WY = TY + 12;
     constexpr int LX = TX + 6, LY = TY + 6;
     extern __shared__ char smem_raw[];
-    R *sU = reinterpret_cast<R *>(smem_raw);  // NK x WY x WX
-    R *sT = sU + NK * WY * WX;                // WY x WX (VISC)
-    R *sL = sT + (VISC ? WY * WX : 0);        // NL x LY x LX
+    R *sU = reinterpret_cast<R *>(smem_raw);  // NK x WY x WX   (U, or q in the perturbation form)
+    R *sT = sU + NK * WY * WX;                // WY x WX (VISC)  (T, or T - Tb)
+    R *sL = sT + (VISC ? WY * WX : 0);        // level-1 fields: LY x LX each
+    constexpr bool SPECIES = NK >= 5;          // passive scalar: Pe diffusion (independent of Re)
+    constexpr int LPSI = VISC ? 5 : 0;         // level-1 slot of the species flux
 
     const int imax = A.imax, jmax = A.jmax, pitch = A.pitch;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
@@ -59,6 +61,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<R> A)
     const Range rx = tile_ranges(i0, i1, imax);
     const Range ry = tile_ranges(j0, j1, jmax);
     const int tid = threadIdx.x;
+    const R gam = R(A.gamma), inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn);
 
     // phase 1: U window (and T = gamma p / rho, rhs.f90:20)
     const int wxn = rx.w1 - rx.w0, wyn = ry.w1 - ry.w0;
@@ -67,8 +70,18 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<R> A)
         if (wx < wxn) {
             const size_t g = (size_t)(ry.w0 + wy) * pitch + (rx.w0 + wx);
 #pragma unroll
-            for (int k = 0; k < NK; ++k) sU[(k * WY + wy) * WX + wx] = A.U[k][g];
-            if (VISC) sT[wy * WX + wx] = A.gamma * sU[(3 * WY + wy) * WX + wx] / sU[(0 * WY + wy) * WX + wx];
+            for (int k = 0; k < NK; ++k) sU[(k * WY + wy) * WX + wx] = R(A.U[k][g]);
+            if constexpr (VISC) {
+                const R q0 = sU[(0 * WY + wy) * WX + wx], q3 = sU[(3 * WY + wy) * WX + wx];
+                if constexpr (PERT) {
+                    // T - Tb = gamma (p' rhob - pb rho') / (rho rhob): no cancellation
+                    const int gy = ry.w0 + wy;
+                    const R rb = R(A.pr.ub[0][gy]), pb = R(A.pr.ub[3][gy]);
+                    sT[wy * WX + wx] = gam * (q3 * rb - pb * q0) / ((rb + q0) * rb);
+                } else {
+                    sT[wy * WX + wx] = gam * q3 / q0;
+                }
+            }
         }
     }
     __syncthreads();
@@ -82,39 +95,57 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<R> A)
         const R *col = sU + k * WY * WX + (gx - ox);
         return [col, oy](int q) { return col[(q - oy) * WX]; };
     };
+    // base-flow y-derivative (perturbation form), 0 otherwise
+    auto dbase = [&](int k, int gy) -> R {
+        if constexpr (PERT) return R(A.pr.dub[k][gy]);
+        return R(0);
+    };
+    auto base = [&](int k, int gy) -> R {
+        if constexpr (PERT) return R(A.pr.ub[k][gy]);
+        return R(0);
+    };
 
     // phase 2: level-1 quantities (viscous stresses, k grad T, species flux)
-    if constexpr (VISC) {
+    if constexpr (VISC || SPECIES) {
+        const Transport<R> tc{R(A.tc.A_mi), R(A.tc.Bt_mi), R(A.tc.Ct_mi), R(A.tc.D_mi), R(A.tc.inv_mi0),
+                              R(A.tc.A_k),  R(A.tc.Bt_k),  R(A.tc.Ct_k),  R(A.tc.D_k),  R(A.tc.inv_k0),
+                              R(A.tc.Tref)};
         const int lxn = rx.l1 - rx.l0, lyn = ry.l1 - ry.l0;
         for (int idx = tid; idx < lyn * LX; idx += 256) {
             const int ly = idx / LX, lx = idx - ly * LX;
             if (lx >= lxn) continue;
             const int gx = rx.l0 + lx, gy = ry.l0 + ly;
-            const R mx = A.metx[gx], my = A.mety[gy];
-            const R ux = drp_deriv<R>(UX(1, gy), gx, imax, A.inv_dm) * mx;
-            const R vx = drp_deriv<R>(UX(2, gy), gx, imax, A.inv_dm) * mx;
-            const R uy = drp_deriv<R>(UYc(1, gx), gy, jmax, A.inv_dn) * my;
-            const R vy = drp_deriv<R>(UYc(2, gx), gy, jmax, A.inv_dn) * my;
+            const R mx = R(A.metx[gx]), my = R(A.mety[gy]);
+            const int o = ly * LX + lx;
+            if constexpr (SPECIES) {
+                const R psx = drp_deriv<R>(UX(4, gy), gx, imax, inv_dm) * mx;
+                const R psy = drp_deriv<R>(UYc(4, gx), gy, jmax, inv_dn) * my + dbase(4, gy);
+                const R rho = UX(0, gy)(gx) + base(0, gy);
+                sL[LPSI * LY * LX + o] = rho * R(A.alpha_cond) * (psx + psy);  // derivs.f90:40
+            }
+            if constexpr (!VISC) continue;
+            const R ux = drp_deriv<R>(UX(1, gy), gx, imax, inv_dm) * mx;
+            const R vx = drp_deriv<R>(UX(2, gy), gx, imax, inv_dm) * mx;
+            const R uy = drp_deriv<R>(UYc(1, gx), gy, jmax, inv_dn) * my + dbase(1, gy);
+            const R vy = drp_deriv<R>(UYc(2, gx), gy, jmax, inv_dn) * my + dbase(2, gy);
             const R *trow = sT + (gy - oy) * WX;
             const R *tcol = sT + (gx - ox);
-            const R Tx = drp_deriv<R>([trow, ox](int q) { return trow[q - ox]; }, gx, imax, A.inv_dm) * mx;
-            const R Ty = drp_deriv<R>([tcol, oy](int q) { return tcol[(q - oy) * WX]; }, gy, jmax, A.inv_dn) * my;
+            const R Tx = drp_deriv<R>([trow, ox](int q) { return trow[q - ox]; }, gx, imax, inv_dm) * mx;
+            R Ty = drp_deriv<R>([tcol, oy](int q) { return tcol[(q - oy) * WX]; }, gy, jmax, inv_dn) * my;
+            R Tval = trow[gx - ox];
+            if constexpr (PERT) {
+                Ty += R(A.pr.dtb[gy]);
+                Tval += R(A.pr.tb[gy]);
+            }
             R mu, kk;
-            transport_eval<R>(A.tc, trow[gx - ox], mu, kk);
+            transport_eval<R>(tc, Tval, mu, kk);
             const R c23mu = R(2.0 / 3.0) * mu;
             const R div = ux + vy;
-            const int o = ly * LX + lx;
             sL[0 * LY * LX + o] = -(c23mu * div) + R(2) * mu * ux;  // eqeuler.f90:37
             sL[1 * LY * LX + o] = mu * (uy + vx);                    // :38
             sL[2 * LY * LX + o] = -(c23mu * div) + R(2) * mu * vy;  // :39
             sL[3 * LY * LX + o] = Tx * kk;                           // derivs.f90:55
             sL[4 * LY * LX + o] = Ty * kk;                           // :54
-            if constexpr (NK >= 5) {
-                const R psx = drp_deriv<R>(UX(4, gy), gx, imax, A.inv_dm) * mx;
-                const R psy = drp_deriv<R>(UYc(4, gx), gy, jmax, A.inv_dn) * my;
-                const R rho = UX(0, gy)(gx);
-                sL[5 * LY * LX + o] = rho * A.alpha_cond * (psx + psy);  // derivs.f90:40
-            }
         }
         __syncthreads();
     }
@@ -123,20 +154,21 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<R> A)
     const int tx = tid & 63;
     const int gx = i0 + tx;
     if (gx >= i1) return;
-    const R mx = A.metx[gx];
+    const R mx = R(A.metx[gx]);
     for (int gy = j0 + (tid >> 6); gy < j1; gy += 4) {
         const bool interior = gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2;
-        const R my = A.mety[gy];
-        R uc[NK], fx[NK], fy[NK];
+        const R my = R(A.mety[gy]);
+        R qc[NK], uc[NK], fx[NK], fy[NK];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
-            uc[k] = UX(k, gy)(gx);
-            fx[k] = drp_deriv<R>(UX(k, gy), gx, imax, A.inv_dm) * mx;
-            fy[k] = drp_deriv<R>(UYc(k, gx), gy, jmax, A.inv_dn) * my;
+            qc[k] = UX(k, gy)(gx);
+            uc[k] = qc[k] + base(k, gy);
+            fx[k] = drp_deriv<R>(UX(k, gy), gx, imax, inv_dm) * mx;
+            fy[k] = drp_deriv<R>(UYc(k, gx), gy, jmax, inv_dn) * my + dbase(k, gy);
         }
         const R rho = uc[0], u = uc[1], v = uc[2], p = uc[3];
         const R irho = R(1) / rho;
-        const R src = (A.amp_src != R(0)) ? A.amp_src * (A.gx[gx] * A.gy[gy]) : R(0);
+        const R src = (A.amp_src != 0.0) ? R(A.amp_src * (A.gx[gx] * A.gy[gy])) : R(0);
         R f[NK];
         // eqeuler.f90:63-66
         f[0] = -(u * fx[0] + rho * fx[1] + v * fy[0] + rho * fy[2]);
@@ -150,54 +182,69 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<R> A)
                 const R *col = sL + l * LY * LX + (gx - lx0);
                 return [col, ly0](int q) { return col[(q - ly0) * LX]; };
             };
-            const R txx_x = drp_deriv<R>(LXa(0), gx, imax, A.inv_dm) * mx;
-            const R txy_x = drp_deriv<R>(LXa(1), gx, imax, A.inv_dm) * mx;
-            const R ktx_x = drp_deriv<R>(LXa(3), gx, imax, A.inv_dm) * mx;
-            const R txy_y = drp_deriv<R>(LYa(1), gy, jmax, A.inv_dn) * my;
-            const R tyy_y = drp_deriv<R>(LYa(2), gy, jmax, A.inv_dn) * my;
-            const R kty_y = drp_deriv<R>(LYa(4), gy, jmax, A.inv_dn) * my;
-            const int o = (gy - ry.l0) * LX + (gx - rx.l0);
+            const R txx_x = drp_deriv<R>(LXa(0), gx, imax, inv_dm) * mx;
+            const R txy_x = drp_deriv<R>(LXa(1), gx, imax, inv_dm) * mx;
+            const R ktx_x = drp_deriv<R>(LXa(3), gx, imax, inv_dm) * mx;
+            const R txy_y = drp_deriv<R>(LYa(1), gy, jmax, inv_dn) * my;
+            const R tyy_y = drp_deriv<R>(LYa(2), gy, jmax, inv_dn) * my;
+            const R kty_y = drp_deriv<R>(LYa(4), gy, jmax, inv_dn) * my;
+            const int o = (gy - ly0) * LX + (gx - lx0);
             const R t0 = sL[0 * LY * LX + o], t1 = sL[1 * LY * LX + o], t2 = sL[2 * LY * LX + o];
-            const R vre = A.inv_Re * irho;
+            const R vre = R(A.inv_Re) * irho;
             // eqeuler.f90:68-92
             f[1] = -(u * fx[1] + irho * fx[3] + v * fy[1] - vre * (txx_x + txy_y));
             f[2] = -(u * fx[2] + v * fy[2] + irho * fy[3] - vre * (txy_x + tyy_y));
-            f[3] = -(A.gamma * p * fx[1] + u * fx[3] + A.gamma * p * fy[2] + v * fy[3] - src -
-                     A.gm1_Re * (fx[1] * t0 + fx[2] * t1 + fy[1] * t1 + fy[2] * t2) -
-                     A.inv_RePr * (ktx_x + kty_y));
-            if constexpr (NK >= 5) {
-                const R psd_x = drp_deriv<R>(LXa(5), gx, imax, A.inv_dm) * mx;
-                const R psd_y = drp_deriv<R>(LYa(5), gy, jmax, A.inv_dn) * my;
-                f[4] = -(u * fx[4] + v * fy[4] - A.inv_Pe * irho * (psd_x + psd_y));  // :94-95
-            }
+            f[3] = -(gam * p * fx[1] + u * fx[3] + gam * p * fy[2] + v * fy[3] - src -
+                     R(A.gm1_Re) * (fx[1] * t0 + fx[2] * t1 + fy[1] * t1 + fy[2] * t2) -
+                     R(A.inv_RePr) * (ktx_x + kty_y));
         } else {
             f[1] = -(u * fx[1] + irho * fx[3] + v * fy[1]);
             f[2] = -(u * fx[2] + v * fy[2] + irho * fy[3]);
-            f[3] = -(A.gamma * p * fx[1] + u * fx[3] + A.gamma * p * fy[2] + v * fy[3] - src);
-            if constexpr (NK >= 5) f[4] = -(u * fx[4] + v * fy[4]);
+            f[3] = -(gam * p * fx[1] + u * fx[3] + gam * p * fy[2] + v * fy[3] - src);
+        }
+        if constexpr (SPECIES) {
+            const int lx0 = rx.l0, ly0 = ry.l0;
+            const R *prow = sL + (LPSI * LY + (gy - ly0)) * LX;
+            const R *pcol = sL + LPSI * LY * LX + (gx - lx0);
+            const R psd_x = drp_deriv<R>([prow, lx0](int q) { return prow[q - lx0]; }, gx, imax, inv_dm) * mx;
+            const R psd_y = drp_deriv<R>([pcol, ly0](int q) { return pcol[(q - ly0) * LX]; }, gy, jmax, inv_dn) * my;
+            f[4] = -(u * fx[4] + v * fy[4] - R(A.inv_Pe) * irho * (psd_x + psd_y));  // eqeuler.f90:94-95
         }
         const size_t g = (size_t)gy * pitch + gx;
         if constexpr (RHS_ONLY) {
 #pragma unroll
-            for (int k = 0; k < NK; ++k) A.dudt[k][g] = f[k];
+            for (int k = 0; k < NK; ++k) A.dudt[k][g] = S(f[k]);
         } else {
             const bool ybuf = gy < A.D || gy >= jmax - A.D;
             const bool xbuf = gx < A.D || gx >= imax - A.D;
-            const R sy = A.sigy[gy], sx = A.sigx[gx];
 #pragma unroll
             for (int k = 0; k < NK; ++k) {
-                R un = uc[k];
-                if (interior) {  // rk.f90:160-167
-                    const R wn = A.first ? A.dt * f[k] : A.alpha * A.W[k][g] + A.dt * f[k];
-                    if (!A.last) A.W[k][g] = wn;
-                    un = uc[k] + A.beta * wn;
+                if constexpr (PERT) {
+                    // stage accumulation of the state in FP64 registers
+                    double qn = double(qc[k]);
+                    if (interior) {  // rk.f90:160-167
+                        const double wn = A.first ? A.dt * double(f[k]) : A.alpha * double(A.W[k][g]) + A.dt * double(f[k]);
+                        if (!A.last) A.W[k][g] = S(wn);
+                        qn = qn + A.beta * wn;
+                    }
+                    // Ub + s (U - Ub) - Ub = s q  (rk.f90:81-108)
+                    if (ybuf) qn = A.sigy[gy] * qn;
+                    if (xbuf) qn = A.sigx[gx] * qn;
+                    A.Uout[k][g] = S(qn);
+                } else {
+                    R un = uc[k];
+                    if (interior) {  // rk.f90:160-167
+                        const R wn = A.first ? R(A.dt) * f[k] : R(A.alpha) * R(A.W[k][g]) + R(A.dt) * f[k];
+                        if (!A.last) A.W[k][g] = S(wn);
+                        un = uc[k] + R(A.beta) * wn;
+                    }
+                    if (ybuf || xbuf) {  // rk.f90:81-108 (y layer first, then x)
+                        const R ub = R(A.base[k][g]);
+                        if (ybuf) un = ub + R(A.sigy[gy]) * (un - ub);
+                        if (xbuf) un = ub + R(A.sigx[gx]) * (un - ub);
+                    }
+                    A.Uout[k][g] = S(un);
                 }
-                if (ybuf || xbuf) {  // rk.f90:81-108 (y layer first, then x)
-                    const R ub = A.base[k][g];
-                    if (ybuf) un = ub + sy * (un - ub);
-                    if (xbuf) un = ub + sx * (un - ub);
-                }
-                A.Uout[k][g] = un;
             }
         }
     }
@@ -207,31 +254,45 @@ template <typename R>
 static size_t stage_smem(int NK, bool visc, int TY)
 {
     const int WX = STX + 12, WY = TY + 12, LX = STX + 6, LY = TY + 6;
-    const int NL = visc ? (NK >= 5 ? 6 : 5) : 0;
+    const int NL = (visc ? 5 : 0) + (NK >= 5 ? 1 : 0);
     return sizeof(R) * ((size_t)NK * WY * WX + (visc ? (size_t)WY * WX : 0) + (size_t)NL * LY * LX);
 }
 
-template <typename R, int NK, bool VISC, bool RHS_ONLY>
-static hipError_t launch_stage_t(const StageArgs<R> &A, hipStream_t s)
+template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
+static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s)
 {
-    constexpr int TY = (NK >= 5 && VISC) ? 8 : 16;
+    using R = S;
+    constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
     dim3 grid((A.imax + STX - 1) / STX, (A.jmax + TY - 1) / TY);
     size_t smem = stage_smem<R>(NK, VISC, TY);
-    auto kern = stage_kernel<R, NK, VISC, TY, RHS_ONLY>;
+    auto kern = stage_kernel<S, R, NK, VISC, TY, RHS_ONLY, PERT>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, grid, dim3(256), smem, s, A);
     return hipGetLastError();
 }
 
-hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, hipStream_t s)
+template <typename S, bool PERT>
+static hipError_t launch_stage_p(const StageArgs<S> &A, int nk, bool visc, bool rhs_only, hipStream_t s)
 {
     if (nk == 4) {
-        if (visc) return rhs_only ? launch_stage_t<double, 4, true, true>(A, s) : launch_stage_t<double, 4, true, false>(A, s);
-        return rhs_only ? launch_stage_t<double, 4, false, true>(A, s) : launch_stage_t<double, 4, false, false>(A, s);
+        if (visc)
+            return rhs_only ? launch_stage_t<S, 4, true, true, PERT>(A, s) : launch_stage_t<S, 4, true, false, PERT>(A, s);
+        return rhs_only ? launch_stage_t<S, 4, false, true, PERT>(A, s) : launch_stage_t<S, 4, false, false, PERT>(A, s);
     }
-    if (visc) return rhs_only ? launch_stage_t<double, 5, true, true>(A, s) : launch_stage_t<double, 5, true, false>(A, s);
-    return rhs_only ? launch_stage_t<double, 5, false, true>(A, s) : launch_stage_t<double, 5, false, false>(A, s);
+    if (visc)
+        return rhs_only ? launch_stage_t<S, 5, true, true, PERT>(A, s) : launch_stage_t<S, 5, true, false, PERT>(A, s);
+    return rhs_only ? launch_stage_t<S, 5, false, true, PERT>(A, s) : launch_stage_t<S, 5, false, false, PERT>(A, s);
+}
+
+hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, hipStream_t s)
+{
+    return launch_stage_p<double, false>(A, nk, visc, rhs_only, s);
+}
+
+hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, hipStream_t s)
+{
+    return launch_stage_p<float, true>(A, nk, visc, rhs_only, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -475,7 +536,7 @@ __global__ __launch_bounds__(64) void filter_x_kernel(FilterArgs<R> F)
     const bool exact = (a <= F.W + 12);
     io.start(exact ? 0 : a - F.W - 6);
     LineEnds ends;
-    filter_line<R, false>(io, F.sigma, F.n, a, b, F.W, ends);
+    filter_line<R, false>(io, R(F.sigma), F.n, a, b, F.W, ends);
 }
 
 // y direction: lanes are columns; rows are contiguous across lanes.
@@ -483,12 +544,13 @@ template <typename R>
 struct YIO {
     const R *in;
     R *out;
+    const double *corr;  // perturbation form: Ly(Ub) - Ub of this row (null in FP64)
     int pitch, col, a, b;
     bool active;
     __device__ __forceinline__ R load(int p) { return active ? in[(size_t)p * pitch + col] : R(0); }
     __device__ __forceinline__ void emit(int e, R v)
     {
-        if (active && e >= a && e < b) out[(size_t)e * pitch + col] = v;
+        if (active && e >= a && e < b) out[(size_t)e * pitch + col] = corr ? R(double(v) + corr[e]) : v;
     }
     __device__ __forceinline__ void finish() {}
 };
@@ -503,65 +565,87 @@ __global__ __launch_bounds__(64) void filter_y_kernel(FilterArgs<R> F)
     YIO<R> io;
     io.in = F.in[k];
     io.out = F.out[k];
+    io.corr = F.corr[k];
     io.pitch = F.pitch;
     io.col = col;
     io.a = a;
     io.b = b;
     io.active = col < F.m;
     LineEnds ends;
-    filter_line<R, true>(io, F.sigma, F.n, a, b, F.W, ends);
+    filter_line<R, true>(io, R(F.sigma), F.n, a, b, F.W, ends);
     // ccpml rows j=1 and j=jmax for columns 2..imax-1 (cc.f90:46-59)
     if (F.edge_bc && io.active && col >= 1 && col <= F.m - 2) {
         R *o = F.out[k];
         const int jm = F.n;
+        const double *cr = F.corr[k];
         if (a == 0) {
-            R val = (k < 3) ? R(2.0) * R(ends.v1) - R(ends.v2) : (k == 3 ? R(1.0) / F.gamma : R(0.0));
-            o[col] = val;
+            double val;
+            if (k < 3) {
+                const double v1 = cr ? ends.v1 + cr[1] : ends.v1, v2 = cr ? ends.v2 + cr[2] : ends.v2;
+                val = 2.0 * v1 - v2 + F.re.bot_res[k];
+            } else {
+                val = F.re.bot_val[k];
+            }
+            o[col] = R(val);
         }
         if (b == jm) {
-            R val = (k < 3) ? R(2.0) * R(ends.vn2) - R(ends.vn3) : (k == 3 ? R(1.0) / F.gamma : R(1.0));
-            o[(size_t)(jm - 1) * F.pitch + col] = val;
+            double val;
+            if (k < 3) {
+                const double v2 = cr ? ends.vn2 + cr[jm - 2] : ends.vn2;
+                const double v3 = cr ? ends.vn3 + cr[jm - 3] : ends.vn3;
+                val = 2.0 * v2 - v3 + F.re.top_res[k];
+            } else {
+                val = F.re.top_val[k];
+            }
+            o[(size_t)(jm - 1) * F.pitch + col] = R(val);
         }
     }
 }
 
-hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s)
+template <typename S>
+static hipError_t launch_filter_t(const FilterArgs<S> &F, bool xdir, hipStream_t s)
 {
     const int nchunks = (F.n + F.L - 1) / F.L;
     if (xdir) {
         dim3 grid(nchunks, (F.m + 63) / 64, F.nk);
-        hipLaunchKernelGGL(filter_x_kernel<double>, grid, dim3(64), 0, s, F);
+        hipLaunchKernelGGL(filter_x_kernel<S>, grid, dim3(64), 0, s, F);
     } else {
         dim3 grid((F.m + 63) / 64, nchunks, F.nk);
-        hipLaunchKernelGGL(filter_y_kernel<double>, grid, dim3(64), 0, s, F);
+        hipLaunchKernelGGL(filter_y_kernel<S>, grid, dim3(64), 0, s, F);
     }
     return hipGetLastError();
 }
+
+hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s) { return launch_filter_t(F, xdir, s); }
+hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s) { return launch_filter_t(F, xdir, s); }
 
 // ---------------------------------------------------------------------------
 // ccpml columns i=1 (inflow) and i=imax (outflow) for every row, with the
 // corner rows j=1, jmax computed from those columns' values (cc.f90:23-59).
 // ---------------------------------------------------------------------------
-__global__ void edge_i_kernel(EdgeArgs E)
+template <typename S>
+__global__ void edge_i_kernel(EdgeArgs<S> E)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int side = blockIdx.y;  // 0: inflow, 1: outflow
     if (j >= E.jmax) return;
     const int imax = E.imax, jmax = E.jmax, pitch = E.pitch;
     const double ig = 1.0 / E.gamma;
-    auto at = [&](int k, int i, int jj) { return E.U[k][(size_t)jj * pitch + i]; };
+    auto at = [&](int k, int i, int jj) { return double(E.U[k][(size_t)jj * pitch + i]); };
+    auto ub = [&](int k, int jj) { return E.ub[k] ? E.ub[k][jj] : 0.0; };
     // value of the column after the inflow/outflow assignment at row jj
+    // (total value minus the base row in the perturbation form)
     auto colval = [&](int k, int jj) -> double {
         if (side == 0) {
             switch (k) {
-            case 0: return E.rhob[jj];
+            case 0: return E.rhob[jj] - ub(0, jj);
             case 1: return 2.0 * at(1, 1, jj) - at(1, 2, jj);
-            case 2: return 0.0;
-            case 3: return ig;
-            default: return E.psib ? E.psib[jj] : 0.0;
+            case 2: return 0.0 - ub(2, jj);
+            case 3: return ig - ub(3, jj);
+            default: return (E.psib ? E.psib[jj] : 0.0) - ub(4, jj);
             }
         }
-        if (k == 3) return ig;
+        if (k == 3) return ig - ub(3, jj);
         return 2.0 * at(k, imax - 2, jj) - at(k, imax - 3, jj);
     };
     const int i = side == 0 ? 0 : imax - 1;
@@ -569,23 +653,27 @@ __global__ void edge_i_kernel(EdgeArgs E)
     for (int k = 0; k < E.nk; ++k) {
         double v;
         if (j == 0) {  // lower boundary overwrites the corner (cc.f90:54-59)
-            v = (k < 3) ? 2.0 * colval(k, 1) - colval(k, 2) : (k == 3 ? ig : 0.0);
+            v = (k < 3) ? 2.0 * colval(k, 1) - colval(k, 2) + E.re.bot_res[k] : E.re.bot_val[k];
         } else if (j == jmax - 1) {  // upper boundary (cc.f90:38-43)
-            v = (k < 3) ? 2.0 * colval(k, jmax - 2) - colval(k, jmax - 3) : (k == 3 ? ig : 1.0);
+            v = (k < 3) ? 2.0 * colval(k, jmax - 2) - colval(k, jmax - 3) + E.re.top_res[k] : E.re.top_val[k];
         } else {
             v = colval(k, j);
         }
         vals[k] = v;
     }
-    for (int k = 0; k < E.nk; ++k) E.U[k][(size_t)j * pitch + i] = vals[k];
+    for (int k = 0; k < E.nk; ++k) E.U[k][(size_t)j * pitch + i] = S(vals[k]);
 }
 
-hipError_t launch_edge_i(const EdgeArgs &E, hipStream_t s)
+template <typename S>
+static hipError_t launch_edge_t(const EdgeArgs<S> &E, hipStream_t s)
 {
     dim3 grid((E.jmax + 255) / 256, 2);
-    hipLaunchKernelGGL(edge_i_kernel, grid, dim3(256), 0, s, E);
+    hipLaunchKernelGGL(edge_i_kernel<S>, grid, dim3(256), 0, s, E);
     return hipGetLastError();
 }
+
+hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s) { return launch_edge_t(E, s); }
+hipError_t launch_edge_i(const EdgeArgs<float> &E, hipStream_t s) { return launch_edge_t(E, s); }
 
 // ---------------------------------------------------------------------------
 // Operator-level kernels for the parity API (one thread per point).
@@ -699,30 +787,34 @@ hipError_t launch_source(double *s_out, int imax, int jmax, double amp, const do
 }
 
 // probes (euler.f90:156-161): gather nk values at npts points
-__global__ void probe_kernel(const double *const *U, int nk, int pitch, int npts, const int *ij, double *out)
+template <typename S>
+__global__ void probe_kernel(const S *const *U, int nk, int pitch, int npts, const int *ij, double *out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npts * nk) return;
     const int pnt = t / nk, k = t - pnt * nk;
     const int i = ij[2 * pnt] - 1, j = ij[2 * pnt + 1] - 1;
-    out[t] = U[k][(size_t)j * pitch + i];
+    out[t] = double(U[k][(size_t)j * pitch + i]);
 }
 
-hipError_t launch_probe(const double *const *U, int nk, int pitch, int npts, const int *ij, double *out,
-                        hipStream_t s)
+template <typename S>
+hipError_t launch_probe(const S *const *U, int nk, int pitch, int npts, const int *ij, double *out, hipStream_t s)
 {
     const int n = npts * nk;
-    hipLaunchKernelGGL(probe_kernel, dim3((n + 255) / 256), dim3(256), 0, s, U, nk, pitch, npts, ij, out);
+    hipLaunchKernelGGL(probe_kernel<S>, dim3((n + 255) / 256), dim3(256), 0, s, U, nk, pitch, npts, ij, out);
     return hipGetLastError();
 }
+template hipError_t launch_probe<double>(const double *const *, int, int, int, const int *, double *, hipStream_t);
+template hipError_t launch_probe<float>(const float *const *, int, int, int, const int *, double *, hipStream_t);
 
 // non-finite scan: flag = 1 if any value is NaN/Inf
-__global__ void finite_kernel(const double *const *U, int nk, int imax, int jmax, int pitch, int *flag)
+template <typename S>
+__global__ void finite_kernel(const S *const *U, int nk, int imax, int jmax, int pitch, int *flag)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= imax) return;
     for (int k = 0; k < nk; ++k) {
-        const double v = U[k][(size_t)j * pitch + i];
+        const S v = U[k][(size_t)j * pitch + i];
         if (!isfinite(v)) {
             atomicOr(flag, 1);
             return;
@@ -730,12 +822,15 @@ __global__ void finite_kernel(const double *const *U, int nk, int imax, int jmax
     }
 }
 
-hipError_t launch_finite(const double *const *U, int nk, int imax, int jmax, int pitch, int *flag, hipStream_t s)
+template <typename S>
+hipError_t launch_finite(const S *const *U, int nk, int imax, int jmax, int pitch, int *flag, hipStream_t s)
 {
     dim3 grid((imax + 127) / 128, jmax);
-    hipLaunchKernelGGL(finite_kernel, grid, dim3(128), 0, s, U, nk, imax, jmax, pitch, flag);
+    hipLaunchKernelGGL(finite_kernel<S>, grid, dim3(128), 0, s, U, nk, imax, jmax, pitch, flag);
     return hipGetLastError();
 }
+template hipError_t launch_finite<double>(const double *const *, int, int, int, int, int *, hipStream_t);
+template hipError_t launch_finite<float>(const float *const *, int, int, int, int, int *, hipStream_t);
 
 // fill kernel (graph-safe zeroing)
 __global__ void fill_kernel(double *p, size_t n, double v)
