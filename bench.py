@@ -60,6 +60,19 @@ def cpu_baseline(case, steps: int):
                       f"OpenMP build of oracle/ns_oracle.c ({dt:.1f} s)"}
 
 
+def copy_probe(mib: int):
+    """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE
+    of this kernel calibrate the counters against bytes actually moved."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_MiB": mib, "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +84,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "fp32pert"], default="fp64")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)

@@ -131,6 +131,15 @@ int32_t ns_ccpml(ns_ctx *ctx, double *U);
 /* rk.f90:171-193 acousticfont(ct,iter): s_acou(imax,jmax) */
 int32_t ns_acousticfont(ns_ctx *ctx, double ct, int32_t iter, double *s_acou);
 
+/* tuning ---------------------------------------------------------------------- */
+/* Time the stage kernel's block-size variants (256, 512, 1024 threads on the
+   same LDS tile; results are bitwise identical) on this context's workload and
+   keep the fastest.  Runs by itself before the first step; call it between
+   steps only.  variant (may be NULL) gets the choice 0..2, ms[3] (may be
+   NULL) the average time of each variant (-1 if it could not launch).
+   No reference counterpart: the reference has no tunable kernels. */
+int32_t ns_autotune(ns_ctx *ctx, int32_t *variant, double *ms);
+
 /* measurement ----------------------------------------------------------------- */
 /* average device time (ms) of each kernel class over the last ns_rk_steps
    call with timing enabled: names "rhs", "filter_x", "filter_y", "bc" */

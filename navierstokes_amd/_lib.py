@@ -78,6 +78,7 @@ SIGNATURES = {
     "ns_filter_y": (C.c_int32, [_V, _DP]),
     "ns_ccpml": (C.c_int32, [_V, _DP]),
     "ns_acousticfont": (C.c_int32, [_V, C.c_double, C.c_int32, _DP]),
+    "ns_autotune": (C.c_int32, [_V, _IP, _DP]),
     "ns_enable_timing": (C.c_int32, [_V, C.c_int32]),
     "ns_kernel_time": (C.c_int32, [_V, C.c_char_p, _DP, _IP]),
 }

@@ -41,19 +41,38 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build libnsamd.so)")
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile libnsamd.so for gfx950 (always recompiles)."""
-    srcs = [str(CSRC / s) for s in SOURCES]
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-I", str(CSRC), *srcs, "-o", str(tmp)]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"hipcc failed with status {res.returncode}")
-    if verbose and res.stderr.strip():
-        sys.stderr.write(res.stderr)
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile libnsamd.so for gfx950 (always recompiles).
+
+    ns_kernels.hip is compiled twice: the FP64 path with -ffp-contract=off
+    (reference operation order, no fused multiply-adds) and the FP32
+    perturbation form with contraction."""
+    cc = hipcc()
+    inc = ["-I", str(ROOT / "include"), "-I", str(CSRC)]
+    base = [f for f in HIPCC_FLAGS if f != "-shared"]
+    bdir = PKG / "_build"
+    bdir.mkdir(exist_ok=True)
+    objs = []
+    units = [
+        ("ns_kernels.hip", "ns_kernels_f64.o", ["-DNS_KERNELS_F64", "-ffp-contract=off"]),
+        ("ns_kernels.hip", "ns_kernels_f32.o", ["-DNS_KERNELS_F32"]),
+        ("ns_api.hip", "ns_api.o", []),
+    ]
+    for src, obj, extra in units:
+        out = bdir / obj
+        _run([cc, *base, *extra, *inc, "-c", str(CSRC / src), "-o", str(out)], verbose)
+        objs.append(str(out))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -20,8 +20,8 @@
 #include "ns_kernels.h"
 
 namespace nsamd {
-hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, hipStream_t s);
-hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, hipStream_t s);
+hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s);
+hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s);
 hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s);
 hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s);
@@ -94,6 +94,8 @@ struct ns_ctx {
     bool have_profiles = false, have_base = false, have_state = false;
     int filt_L = 256, filt_W = 40;
     hipGraphExec_t graph = nullptr;
+    int stage_variant = -1;       // block-size variant of the stage kernel (-1: not tuned)
+    double tune_ms[3] = {0, 0, 0};
     bool timing = false;
     hipEvent_t ev[2 * 24] = {};
     double tsum[TC_N] = {};
@@ -301,7 +303,7 @@ int enqueue_step_t(ns_ctx *c, int iter, bool timed)
         };
         StageArgs<S> A = stage_args<S>(c, irk, S0, S1, stage_amp(P, irk, iter), pert);
         tick(0);
-        HIPCHK(launch_stage(A, P.nk, visc, false, c->stream));
+        HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream));
         tick(1);
         HIPCHK(launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, c->stream));
         tick(2);
@@ -325,6 +327,47 @@ int enqueue_step_t(ns_ctx *c, int iter, bool timed)
 int enqueue_step(ns_ctx *c, int iter, bool timed)
 {
     return c->pert ? enqueue_step_t<float>(c, iter, timed) : enqueue_step_t<double>(c, iter, timed);
+}
+
+// Autotuner: time each block-size variant of the stage kernel on the live
+// workload and keep the fastest.  It runs stage 1 (which does not read W)
+// from S0 into S1, so it leaves the state untouched when called between steps.
+template <typename S>
+int autotune_stage_t(ns_ctx *c)
+{
+    const ns_params &P = c->P;
+    const bool visc = !(P.flags & NS_FLAG_INVISCID);
+    S *S0 = static_cast<S *>(c->S[0]), *S1 = static_cast<S *>(c->S[1]);
+    const StageArgs<S> A = stage_args<S>(c, 0, S0, S1, 0.0, c->pert);
+    int best = 0;
+    double best_ms = 1e300;
+    const int reps = 3;
+    for (int v = 0; v < 3; ++v) {
+        hipError_t e = launch_stage(A, P.nk, visc, false, v, c->stream);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            c->tune_ms[v] = -1.0;
+            continue;
+        }
+        HIPCHK(hipEventRecord(c->ev[0], c->stream));
+        for (int r = 0; r < reps; ++r) HIPCHK(launch_stage(A, P.nk, visc, false, v, c->stream));
+        HIPCHK(hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        c->tune_ms[v] = ms / reps;
+        if (ms / reps < best_ms) {
+            best_ms = ms / reps;
+            best = v;
+        }
+    }
+    c->stage_variant = best;
+    return NS_OK;
+}
+
+int autotune_stage(ns_ctx *c)
+{
+    return c->pert ? autotune_stage_t<float>(c) : autotune_stage_t<double>(c);
 }
 
 int check_ready(ns_ctx *c)
@@ -785,6 +828,7 @@ int32_t ns_rk_euler(ns_ctx *c, int32_t iter)
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->stage_variant < 0 && (rc = autotune_stage(c))) return rc;
     rc = enqueue_step(c, iter, false);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -796,6 +840,7 @@ int32_t ns_rk_steps(ns_ctx *c, int32_t iter0, int32_t nsteps)
     int rc = check_ready(c);
     if (rc) return rc;
     if (nsteps < 0) return fail(NS_ERR_PARAM, "nsteps < 0");
+    if (c->stage_variant < 0 && (rc = autotune_stage(c))) return rc;
     if (c->timing || c->P.amplitude != 0.0) {
         // per-step launches: the source amplitude changes with iter
         for (int s = 0; s < nsteps; ++s) {
@@ -868,7 +913,8 @@ int32_t ns_rhs(ns_ctx *c, double *dUdt, const double *U, double ct, int32_t iter
     double amp = P.amplitude == 0.0 ? 0.0 : P.amplitude * std::sin(P.wf * (((double)iter + ct) * P.dt));
     StageArgs<double> A = stage_args<double>(c, 0, in, outb, amp, false);
     for (int k = 0; k < P.nk; ++k) A.dudt[k] = plane_ptr(outb, c, k);
-    HIPCHK(launch_stage(A, P.nk, !(P.flags & NS_FLAG_INVISCID), true, c->stream));
+    HIPCHK(launch_stage(A, P.nk, !(P.flags & NS_FLAG_INVISCID), true, c->stage_variant < 0 ? 0 : c->stage_variant,
+                        c->stream));
     return download_aos(c, outb, dUdt, P.nk);
 }
 
@@ -946,6 +992,21 @@ int32_t ns_acousticfont(ns_ctx *c, double ct, int32_t iter, double *s_acou)
     HIPCHK(launch_source(c->work, P.imax, P.jmax, amp, c->gx, c->gy, c->stream));
     HIPCHK(hipMemcpyAsync(s_acou, c->work, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int32_t ns_autotune(ns_ctx *c, int32_t *variant, double *ms)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (c->graph) {
+        HIPCHK(hipGraphExecDestroy(c->graph));
+        c->graph = nullptr;
+    }
+    if ((rc = autotune_stage(c))) return rc;
+    if (variant) *variant = c->stage_variant;
+    if (ms)
+        for (int v = 0; v < 3; ++v) ms[v] = c->tune_ms[v];
     return NS_OK;
 }
 

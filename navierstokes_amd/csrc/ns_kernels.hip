@@ -8,6 +8,11 @@
 //   filter_y_kernel: lddfiltery (filtering.f90:98-190) + ccpml rows j=1,jmax
 //   edge_i_kernel  : ccpml columns i=1,imax incl. corners (cc.f90:6-68)
 // State planes are structure-of-arrays, row-major with i fastest (pitch).
+//
+// This file is compiled twice: with NS_KERNELS_F64 and -ffp-contract=off
+// (the FP64 path keeps the reference's operation order, no fused
+// multiply-adds, so it rounds like the reference), and with NS_KERNELS_F32
+// (the FP32 perturbation form, contraction allowed).
 #include <hip/hip_runtime.h>
 
 #include "ns_kernels.h"
@@ -42,8 +47,8 @@ __device__ __forceinline__ Range tile_ranges(int t0, int t1, int n)
     return r;
 }
 
-template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT>
-__global__ __launch_bounds__(256) void stage_kernel(StageArgs<S> A)
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT>
+__global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
 {
     constexpr int TX = STX;
     constexpr int WX = TX + 12, WY = TY + 12;
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<S> A)
 
     // phase 1: U window (and T = gamma p / rho, rhs.f90:20)
     const int wxn = rx.w1 - rx.w0, wyn = ry.w1 - ry.w0;
-    for (int idx = tid; idx < wyn * WX; idx += 256) {
+    for (int idx = tid; idx < wyn * WX; idx += NT) {
         const int wy = idx / WX, wx = idx - wy * WX;
         if (wx < wxn) {
             const size_t g = (size_t)(ry.w0 + wy) * pitch + (rx.w0 + wx);
@@ -111,7 +116,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<S> A)
                               R(A.tc.A_k),  R(A.tc.Bt_k),  R(A.tc.Ct_k),  R(A.tc.D_k),  R(A.tc.inv_k0),
                               R(A.tc.Tref)};
         const int lxn = rx.l1 - rx.l0, lyn = ry.l1 - ry.l0;
-        for (int idx = tid; idx < lyn * LX; idx += 256) {
+        for (int idx = tid; idx < lyn * LX; idx += NT) {
             const int ly = idx / LX, lx = idx - ly * LX;
             if (lx >= lxn) continue;
             const int gx = rx.l0 + lx, gy = ry.l0 + ly;
@@ -155,7 +160,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<S> A)
     const int gx = i0 + tx;
     if (gx >= i1) return;
     const R mx = R(A.metx[gx]);
-    for (int gy = j0 + (tid >> 6); gy < j1; gy += 4) {
+    for (int gy = j0 + (tid >> 6); gy < j1; gy += NT / 64) {
         const bool interior = gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2;
         const R my = R(A.mety[gy]);
         R qc[NK], uc[NK], fx[NK], fy[NK];
@@ -258,42 +263,62 @@ static size_t stage_smem(int NK, bool visc, int TY)
     return sizeof(R) * ((size_t)NK * WY * WX + (visc ? (size_t)WY * WX : 0) + (size_t)NL * LY * LX);
 }
 
-template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
+template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT>
 static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s)
 {
     using R = S;
     constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
     dim3 grid((A.imax + STX - 1) / STX, (A.jmax + TY - 1) / TY);
     size_t smem = stage_smem<R>(NK, VISC, TY);
-    auto kern = stage_kernel<S, R, NK, VISC, TY, RHS_ONLY, PERT>;
+    auto kern = stage_kernel<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, grid, dim3(256), smem, s, A);
+    hipLaunchKernelGGL(kern, grid, dim3(NT), smem, s, A);
     return hipGetLastError();
 }
 
+// block-size variants of the stage kernel (same tile, same arithmetic):
+// 0 -> 256 threads, 1 -> 512, 2 -> 1024
+template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
+static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s)
+{
+    switch (variant) {
+    case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512>(A, s);
+    case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 1024>(A, s);
+    default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 256>(A, s);
+    }
+}
+
 template <typename S, bool PERT>
-static hipError_t launch_stage_p(const StageArgs<S> &A, int nk, bool visc, bool rhs_only, hipStream_t s)
+static hipError_t launch_stage_p(const StageArgs<S> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s)
 {
     if (nk == 4) {
         if (visc)
-            return rhs_only ? launch_stage_t<S, 4, true, true, PERT>(A, s) : launch_stage_t<S, 4, true, false, PERT>(A, s);
-        return rhs_only ? launch_stage_t<S, 4, false, true, PERT>(A, s) : launch_stage_t<S, 4, false, false, PERT>(A, s);
+            return rhs_only ? launch_stage_v<S, 4, true, true, PERT>(A, variant, s)
+                            : launch_stage_v<S, 4, true, false, PERT>(A, variant, s);
+        return rhs_only ? launch_stage_v<S, 4, false, true, PERT>(A, variant, s)
+                        : launch_stage_v<S, 4, false, false, PERT>(A, variant, s);
     }
     if (visc)
-        return rhs_only ? launch_stage_t<S, 5, true, true, PERT>(A, s) : launch_stage_t<S, 5, true, false, PERT>(A, s);
-    return rhs_only ? launch_stage_t<S, 5, false, true, PERT>(A, s) : launch_stage_t<S, 5, false, false, PERT>(A, s);
+        return rhs_only ? launch_stage_v<S, 5, true, true, PERT>(A, variant, s)
+                        : launch_stage_v<S, 5, true, false, PERT>(A, variant, s);
+    return rhs_only ? launch_stage_v<S, 5, false, true, PERT>(A, variant, s)
+                    : launch_stage_v<S, 5, false, false, PERT>(A, variant, s);
 }
 
-hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, hipStream_t s)
+#if defined(NS_KERNELS_F64)
+hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s)
 {
-    return launch_stage_p<double, false>(A, nk, visc, rhs_only, s);
+    return launch_stage_p<double, false>(A, nk, visc, rhs_only, variant, s);
 }
+#endif
 
-hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, hipStream_t s)
+#if defined(NS_KERNELS_F32)
+hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s)
 {
-    return launch_stage_p<float, true>(A, nk, visc, rhs_only, s);
+    return launch_stage_p<float, true>(A, nk, visc, rhs_only, variant, s);
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // In-place selective filter as a chunked sweep with look-back.
@@ -616,8 +641,12 @@ static hipError_t launch_filter_t(const FilterArgs<S> &F, bool xdir, hipStream_t
     return hipGetLastError();
 }
 
+#if defined(NS_KERNELS_F64)
 hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s) { return launch_filter_t(F, xdir, s); }
+#endif
+#if defined(NS_KERNELS_F32)
 hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s) { return launch_filter_t(F, xdir, s); }
+#endif
 
 // ---------------------------------------------------------------------------
 // ccpml columns i=1 (inflow) and i=imax (outflow) for every row, with the
@@ -672,8 +701,14 @@ static hipError_t launch_edge_t(const EdgeArgs<S> &E, hipStream_t s)
     return hipGetLastError();
 }
 
+#if defined(NS_KERNELS_F64)
 hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s) { return launch_edge_t(E, s); }
+#endif
+#if defined(NS_KERNELS_F32)
 hipError_t launch_edge_i(const EdgeArgs<float> &E, hipStream_t s) { return launch_edge_t(E, s); }
+#endif
+
+#if defined(NS_KERNELS_F64)
 
 // ---------------------------------------------------------------------------
 // Operator-level kernels for the parity API (one thread per point).
@@ -786,6 +821,8 @@ hipError_t launch_source(double *s_out, int imax, int jmax, double amp, const do
     return hipGetLastError();
 }
 
+#endif  // NS_KERNELS_F64 (operator-level kernels)
+
 // probes (euler.f90:156-161): gather nk values at npts points
 template <typename S>
 __global__ void probe_kernel(const S *const *U, int nk, int pitch, int npts, const int *ij, double *out)
@@ -804,8 +841,12 @@ hipError_t launch_probe(const S *const *U, int nk, int pitch, int npts, const in
     hipLaunchKernelGGL(probe_kernel<S>, dim3((n + 255) / 256), dim3(256), 0, s, U, nk, pitch, npts, ij, out);
     return hipGetLastError();
 }
+#if defined(NS_KERNELS_F64)
 template hipError_t launch_probe<double>(const double *const *, int, int, int, const int *, double *, hipStream_t);
+#endif
+#if defined(NS_KERNELS_F32)
 template hipError_t launch_probe<float>(const float *const *, int, int, int, const int *, double *, hipStream_t);
+#endif
 
 // non-finite scan: flag = 1 if any value is NaN/Inf
 template <typename S>
@@ -829,9 +870,14 @@ hipError_t launch_finite(const S *const *U, int nk, int imax, int jmax, int pitc
     hipLaunchKernelGGL(finite_kernel<S>, grid, dim3(128), 0, s, U, nk, imax, jmax, pitch, flag);
     return hipGetLastError();
 }
+#if defined(NS_KERNELS_F64)
 template hipError_t launch_finite<double>(const double *const *, int, int, int, int, int *, hipStream_t);
+#endif
+#if defined(NS_KERNELS_F32)
 template hipError_t launch_finite<float>(const float *const *, int, int, int, int, int *, hipStream_t);
+#endif
 
+#if defined(NS_KERNELS_F64)
 // fill kernel (graph-safe zeroing)
 __global__ void fill_kernel(double *p, size_t n, double v)
 {
@@ -844,5 +890,7 @@ hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s)
     hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
+
+#endif  // NS_KERNELS_F64
 
 }  // namespace nsamd

@@ -178,7 +178,14 @@ class Solver:
         _lib.check(self.lib.ns_acousticfont(self.ctx, float(ct), int(iter), _lib.dptr(s)))
         return s
 
-    # measurement --------------------------------------------------------------
+    # tuning / measurement -----------------------------------------------------
+    def autotune(self):
+        """Pick the fastest stage-kernel block size; returns (choice, ms[3])."""
+        v = C.c_int32()
+        ms = np.zeros(3)
+        _lib.check(self.lib.ns_autotune(self.ctx, C.byref(v), _lib.dptr(ms)))
+        return v.value, ms
+
     def enable_timing(self, on: bool = True):
         _lib.check(self.lib.ns_enable_timing(self.ctx, int(on)))
 
