@@ -131,6 +131,26 @@ int32_t ns_ccpml(ns_ctx *ctx, double *U);
 /* rk.f90:171-193 acousticfont(ct,iter): s_acou(imax,jmax) */
 int32_t ns_acousticfont(ns_ctx *ctx, double ct, int32_t iter, double *s_acou);
 
+/* batched cases (no reference counterpart: the reference runs one case per
+   process) ------------------------------------------------------------------ */
+/* Many independent cases of any grid sizes advanced together.  Each kernel
+   class runs as one launch over a tile table that spans every case, and one
+   captured HIP graph replays a whole step of the batch.  All cases share nk,
+   precision and flags; the acoustic source must be off (amplitude 0). */
+typedef struct ns_batch ns_batch;
+int32_t ns_batch_create(const ns_params *params, int32_t ncases, ns_batch **out);
+/* profiles, base flow and initial state of case c (same meaning as
+   ns_set_profiles / ns_set_base / ns_put_state) */
+int32_t ns_batch_set_case(ns_batch *b, int32_t c, const double *meshx, const double *meshy,
+                          const double *sigmax, const double *sigmay, const double *m,
+                          const double *n, const double *rhob, const double *psib,
+                          const double *Ub, const double *U0);
+int32_t ns_batch_steps(ns_batch *b, int32_t iter0, int32_t nsteps);
+int32_t ns_batch_get_state(ns_batch *b, int32_t c, double *U);
+/* stage-kernel block size for the whole batch, as ns_autotune */
+int32_t ns_batch_autotune(ns_batch *b, int32_t *variant, double *ms);
+int32_t ns_batch_destroy(ns_batch *b);
+
 /* tuning ---------------------------------------------------------------------- */
 /* Time the stage kernel's block-size variants (256, 512, 1024 threads on the
    same LDS tile; results are bitwise identical) on this context's workload and

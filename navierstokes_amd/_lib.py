@@ -81,6 +81,12 @@ SIGNATURES = {
     "ns_autotune": (C.c_int32, [_V, _IP, _DP]),
     "ns_enable_timing": (C.c_int32, [_V, C.c_int32]),
     "ns_kernel_time": (C.c_int32, [_V, C.c_char_p, _DP, _IP]),
+    "ns_batch_create": (C.c_int32, [C.POINTER(NSParams), C.c_int32, C.POINTER(_V)]),
+    "ns_batch_set_case": (C.c_int32, [_V, C.c_int32] + [_DP] * 10),
+    "ns_batch_steps": (C.c_int32, [_V, C.c_int32, C.c_int32]),
+    "ns_batch_get_state": (C.c_int32, [_V, C.c_int32, _DP]),
+    "ns_batch_autotune": (C.c_int32, [_V, _IP, _DP]),
+    "ns_batch_destroy": (C.c_int32, [_V]),
 }
 
 _lib = None

@@ -276,6 +276,31 @@ def pulse_vortex_case(N: int = 4096, D: int = 40, M: float = 0.5, Re_pulse: floa
                 Re=Re_pulse / half_width, Pr=Pr)
 
 
+def batch_sweep_cases(ncases: int = 256, seed: int = 0, sizes=(128, 192, 256, 384, 512), D: int = 20,
+                      M_range=(0.0, 0.6), amp_range=(1e-4, 1e-2), hw_range=(2.0, 6.0),
+                      max_points: int | None = None) -> list:
+    """Seeded sweep of independent 2-D Euler Gaussian-pulse cases: square
+    grids drawn uniformly from `sizes`, uniform mean flow M ~ U(M_range),
+    amplitude log-uniform in amp_range (relative to p0), half-width
+    ~ U(hw_range) grid spacings.  `max_points` (optional) redraws sizes so
+    the total point count stays below it."""
+    rng = np.random.default_rng(seed)
+    out = []
+    total = 0
+    for c in range(ncases):
+        N = int(rng.choice(sizes))
+        M = float(rng.uniform(*M_range))
+        amp = float(np.exp(rng.uniform(np.log(amp_range[0]), np.log(amp_range[1]))))
+        hw = float(rng.uniform(*hw_range))
+        if max_points is not None and total + N * N > max_points:
+            N = int(min(sizes))
+        total += N * N
+        case = acoustic_pulse_case(N, D=D, amp_rel=amp, half_width=hw, u0=M)
+        case.name = f"sweep-{c}-{N}-M{M:.3f}"
+        out.append(case)
+    return out
+
+
 def random_case(imax: int, jmax: int, nk: int = 4, D: int = 6, seed: int = 0, Re: float = 500.0,
                 amp: float = 1e-2, dt: float = 0.2, sigmaf: float = 0.2, inviscid: bool = False,
                 mesh_random: bool = True) -> Case:

@@ -26,6 +26,16 @@ hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s);
 hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<float> &E, hipStream_t s);
+hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
+                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s);
+hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
+                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s);
+hipError_t launch_filter_batch(const FilterArgs<double> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
+                               hipStream_t s);
+hipError_t launch_filter_batch(const FilterArgs<float> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
+                               hipStream_t s);
+hipError_t launch_edge_batch(const EdgeArgs<double> *args, int ncases, hipStream_t s);
+hipError_t launch_edge_batch(const EdgeArgs<float> *args, int ncases, hipStream_t s);
 hipError_t launch_deriv(const double *const *in, double *const *out, int nk, int imax, int jmax, int pitch, bool ydir,
                         const double *met, double inv_h, hipStream_t s);
 hipError_t launch_transport(const double *T, double *MI, double *K1, size_t n, const Transport<double> &tc,
@@ -237,6 +247,7 @@ StageArgs<S> stage_args(ns_ctx *c, int irk, S *Uin, S *Uout, double amp, bool pe
     A.tc = make_transport(P);
     A.first = irk == 0;
     A.last = irk == 5;
+    A.fast_ok = (P.dm == 1.0 && P.dn == 1.0 && !(P.flags & NS_FLAG_METRIC_EVERYWHERE)) ? 1 : 0;
     return A;
 }
 
@@ -1031,6 +1042,276 @@ int32_t ns_kernel_time(ns_ctx *c, const char *name, double *avg_ms, int32_t *cou
             return NS_OK;
         }
     return fail(NS_ERR_PARAM, "unknown kernel class '%s'", name);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched cases: one context per case holds the storage and the set-up; the
+// batch owns per-(stage, case) argument blocks and tile tables in device
+// memory so that each kernel class is a single launch over all cases.
+// ---------------------------------------------------------------------------
+struct ns_batch {
+    std::vector<ns_ctx *> cs;
+    int n = 0, nk = 0;
+    bool pert = false, visc = true;
+    hipStream_t stream = nullptr;
+    void *sargs = nullptr, *fxargs = nullptr, *fyargs = nullptr, *eargs = nullptr;
+    int4 *stiles = nullptr, *gtiles = nullptr, *fxtiles = nullptr, *fytiles = nullptr;
+    int nst = 0, ngt = 0, nfx = 0, nfy = 0;
+    bool prepared = false;
+    int variant = -1;
+    double tune_ms[3] = {0, 0, 0};
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void batch_release(ns_batch *b)
+{
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (void *p : {b->sargs, b->fxargs, b->fyargs, b->eargs, (void *)b->stiles, (void *)b->gtiles, (void *)b->fxtiles,
+                    (void *)b->fytiles})
+        if (p) (void)hipFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (ns_ctx *c : b->cs)
+        if (c) {
+            release(c);
+            delete c;
+        }
+}
+
+template <typename T>
+int upload_vec(hipStream_t s, void **dst, const std::vector<T> &v)
+{
+    HIPCHK(hipMalloc(dst, v.size() * sizeof(T)));
+    HIPCHK(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return NS_OK;
+}
+
+template <typename S>
+int batch_prepare_t(ns_batch *b)
+{
+    const int n = b->n;
+    std::vector<StageArgs<S>> sa((size_t)6 * n);
+    std::vector<FilterArgs<S>> fx(n), fy(n);
+    std::vector<EdgeArgs<S>> ea(n);
+    std::vector<int4> st, gt, tx, ty;
+    const int TY = (sizeof(S) == 8 && b->nk >= 5 && b->visc) ? 8 : 16;
+    for (int c = 0; c < n; ++c) {
+        ns_ctx *x = b->cs[c];
+        S *S0 = static_cast<S *>(x->S[0]), *S1 = static_cast<S *>(x->S[1]), *S2 = static_cast<S *>(x->S[2]);
+        for (int irk = 0; irk < 6; ++irk) sa[(size_t)irk * n + c] = stage_args<S>(x, irk, S0, S1, 0.0, b->pert);
+        fx[c] = filter_args<S>(x, true, S1, S2, false, b->pert);
+        fy[c] = filter_args<S>(x, false, S2, S0, true, b->pert);
+        ea[c] = edge_args<S>(x, S0, b->pert);
+        const int imax = x->P.imax, jmax = x->P.jmax, L = x->filt_L;
+        int fx0, fx1, fy0, fy1;
+        const StageArgs<S> &A0 = sa[c];
+        fast_range(A0.fast_ok, A0.D, imax, 64, fx0, fx1);
+        fast_range(A0.fast_ok, A0.D, jmax, TY, fy0, fy1);
+        for (int by = 0; by < (jmax + TY - 1) / TY; ++by)
+            for (int bx = 0; bx < (imax + 63) / 64; ++bx) {
+                const bool fast = bx >= fx0 && bx < fx1 && by >= fy0 && by < fy1;
+                (fast ? st : gt).push_back(make_int4(c, bx, by, 0));
+            }
+        for (int rb = 0; rb < (jmax + 63) / 64; ++rb)
+            for (int ch = 0; ch < (imax + L - 1) / L; ++ch) tx.push_back(make_int4(c, ch, rb, 0));
+        for (int ch = 0; ch < (jmax + L - 1) / L; ++ch)
+            for (int cb = 0; cb < (imax + 63) / 64; ++cb) ty.push_back(make_int4(c, cb, ch, 0));
+    }
+    int rc;
+    if ((rc = upload_vec(b->stream, &b->sargs, sa)) || (rc = upload_vec(b->stream, &b->fxargs, fx)) ||
+        (rc = upload_vec(b->stream, &b->fyargs, fy)) || (rc = upload_vec(b->stream, &b->eargs, ea)) ||
+        (rc = upload_vec(b->stream, (void **)&b->stiles, st)) || (rc = upload_vec(b->stream, (void **)&b->gtiles, gt)) ||
+        (rc = upload_vec(b->stream, (void **)&b->fxtiles, tx)) ||
+        (rc = upload_vec(b->stream, (void **)&b->fytiles, ty)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->nst = (int)st.size();
+    b->ngt = (int)gt.size();
+    b->nfx = (int)tx.size();
+    b->nfy = (int)ty.size();
+    b->prepared = true;
+    return NS_OK;
+}
+
+template <typename S>
+int batch_enqueue_step_t(ns_batch *b)
+{
+    for (int irk = 0; irk < 6; ++irk) {
+        HIPCHK(launch_stage_batch(static_cast<const StageArgs<S> *>(b->sargs), b->stiles, b->nst, b->gtiles, b->ngt, irk,
+                                  b->n, b->nk, b->visc, b->variant < 0 ? 0 : b->variant, b->stream));
+        HIPCHK(launch_filter_batch(static_cast<const FilterArgs<S> *>(b->fxargs), b->fxtiles, b->nfx, b->nk, true,
+                                   b->stream));
+        HIPCHK(launch_filter_batch(static_cast<const FilterArgs<S> *>(b->fyargs), b->fytiles, b->nfy, b->nk, false,
+                                   b->stream));
+        HIPCHK(launch_edge_batch(static_cast<const EdgeArgs<S> *>(b->eargs), b->n, b->stream));
+    }
+    return NS_OK;
+}
+
+int batch_enqueue_step(ns_batch *b)
+{
+    return b->pert ? batch_enqueue_step_t<float>(b) : batch_enqueue_step_t<double>(b);
+}
+
+// time the stage kernel's variants on stage 1 of the batch (W dead, S1 scratch)
+template <typename S>
+int batch_autotune_t(ns_batch *b)
+{
+    int best = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < 3; ++v) {
+        const StageArgs<S> *sa = static_cast<const StageArgs<S> *>(b->sargs);
+        hipError_t e = launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 0, b->n, b->nk, b->visc, v, b->stream);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            b->tune_ms[v] = -1.0;
+            continue;
+        }
+        HIPCHK(hipEventRecord(b->ev0, b->stream));
+        for (int r = 0; r < 3; ++r)
+            HIPCHK(launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 0, b->n, b->nk, b->visc, v, b->stream));
+        HIPCHK(hipEventRecord(b->ev1, b->stream));
+        HIPCHK(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->tune_ms[v] = ms / 3;
+        if (ms / 3 < best_ms) {
+            best_ms = ms / 3;
+            best = v;
+        }
+    }
+    b->variant = best;
+    return NS_OK;
+}
+
+int batch_ready(ns_batch *b)
+{
+    if (!b) return fail(NS_ERR_PARAM, "null batch");
+    for (int c = 0; c < b->n; ++c)
+        if (!b->cs[c]->have_profiles || !b->cs[c]->have_base || !b->cs[c]->have_state)
+            return fail(NS_ERR_STATE, "case %d has not been set (ns_batch_set_case)", c);
+    int rc;
+    if (!b->prepared && (rc = b->pert ? batch_prepare_t<float>(b) : batch_prepare_t<double>(b))) return rc;
+    if (b->variant < 0 && (rc = b->pert ? batch_autotune_t<float>(b) : batch_autotune_t<double>(b))) return rc;
+    return NS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ns_batch_create(const ns_params *params, int32_t ncases, ns_batch **out)
+{
+    if (!params || !out || ncases < 1) return fail(NS_ERR_PARAM, "bad argument");
+    *out = nullptr;
+    for (int c = 0; c < ncases; ++c) {
+        const ns_params &P = params[c];
+        if (P.nk != params[0].nk || P.precision != params[0].precision || P.flags != params[0].flags ||
+            P.device != params[0].device)
+            return fail(NS_ERR_PARAM, "case %d: all cases of a batch share nk, precision, flags and device", c);
+        if (P.amplitude != 0.0) return fail(NS_ERR_PARAM, "case %d: the batched path runs without acoustic source", c);
+    }
+    ns_batch *b = new ns_batch();
+    b->n = ncases;
+    b->nk = params[0].nk;
+    b->pert = params[0].precision == NS_PREC_FP32_PERT;
+    b->visc = !(params[0].flags & NS_FLAG_INVISCID);
+    b->cs.assign(ncases, nullptr);
+    for (int c = 0; c < ncases; ++c) {
+        int rc = ns_create(&params[c], &b->cs[c]);
+        if (rc) {
+            std::string msg = g_err;
+            batch_release(b);
+            delete b;
+            return fail(rc, "case %d: %s", c, msg.c_str());
+        }
+    }
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        batch_release(b);
+        delete b;
+        return fail(NS_ERR_HIP, "batch stream/events: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return NS_OK;
+}
+
+int32_t ns_batch_set_case(ns_batch *b, int32_t c, const double *meshx, const double *meshy, const double *sigmax,
+                          const double *sigmay, const double *m, const double *n, const double *rhob,
+                          const double *psib, const double *Ub, const double *U0)
+{
+    if (!b || c < 0 || c >= b->n) return fail(NS_ERR_PARAM, "bad batch or case index %d", c);
+    ns_ctx *x = b->cs[c];
+    int rc;
+    if ((rc = ns_set_profiles(x, meshx, meshy, sigmax, sigmay, m, n, rhob, psib))) return rc;
+    if ((rc = ns_set_base(x, Ub))) return rc;
+    if ((rc = ns_put_state(x, U0))) return rc;
+    return NS_OK;
+}
+
+int32_t ns_batch_steps(ns_batch *b, int32_t iter0, int32_t nsteps)
+{
+    int rc = batch_ready(b);
+    if (rc) return rc;
+    (void)iter0;  // no source term in the batched path: steps do not depend on iter
+    if (nsteps < 0) return fail(NS_ERR_PARAM, "nsteps < 0");
+    if (!b->graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = batch_enqueue_step(b);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(NS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(NS_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(b->graph, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return NS_OK;
+}
+
+int32_t ns_batch_get_state(ns_batch *b, int32_t c, double *U)
+{
+    if (!b || c < 0 || c >= b->n || !U) return fail(NS_ERR_PARAM, "bad batch, case index or pointer");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return ns_get_state(b->cs[c], U);
+}
+
+int32_t ns_batch_autotune(ns_batch *b, int32_t *variant, double *ms)
+{
+    if (!b) return fail(NS_ERR_PARAM, "null batch");
+    int rc = batch_ready(b);
+    if (rc) return rc;
+    if (b->graph) {
+        HIPCHK(hipGraphExecDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    if ((rc = b->pert ? batch_autotune_t<float>(b) : batch_autotune_t<double>(b))) return rc;
+    if (variant) *variant = b->variant;
+    if (ms)
+        for (int v = 0; v < 3; ++v) ms[v] = b->tune_ms[v];
+    return NS_OK;
+}
+
+int32_t ns_batch_destroy(ns_batch *b)
+{
+    if (!b) return NS_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_release(b);
+    delete b;
+    return NS_OK;
 }
 
 }  // extern "C"

@@ -44,6 +44,7 @@ struct StageArgs {
     double amp_src;  // amplitude*sin(wf*(iter+c)*dt) of this stage (0: no source)
     Transport<double> tc;
     int first, last;  // stage 1 (alpha=0: W not read), stage 6 (W not written)
+    int fast_ok;      // unit grid, region metrics: interior tiles may take the fast path
 };
 
 // ccpml constants of the rows j=1 / j=jmax (cc.f90:38-59): components 0-2
@@ -79,5 +80,16 @@ struct EdgeArgs {
     int imax, jmax, pitch, nk;
     double gamma;
 };
+
+// Range of tiles that may take the fast path (unit grid, no metric or buffer
+// work on the tile or its halo): [b0, b1) along
+// one direction of n points with tile size T.  Empty when fast_ok is 0.
+__host__ __device__ inline void fast_range(int fast_ok, int D, int n, int T, int &b0, int &b1)
+{
+    const int Dm = D > 4 ? D : 4;
+    b0 = (Dm + 3 + T - 1) / T;         // first tile with i0 - 3 >= Dm
+    b1 = (n - Dm - 3 - T) / T + 1;     // past the last tile with i0 + T + 3 <= n - Dm
+    if (!fast_ok || n - Dm - 3 - T < 0 || b1 < b0) b0 = b1 = 0;
+}
 
 }  // namespace nsamd

@@ -47,8 +47,23 @@ __device__ __forceinline__ Range tile_ranges(int t0, int t1, int n)
     return r;
 }
 
-template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT>
-__global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
+// Interior DRP derivative on a unit grid (dm = dn = 1): the reference's
+// ((a1 d1)/1 + (a2 d2)/1) + (a3 d3)/1 without the exact divisions by 1.
+template <typename R, typename F>
+__device__ __forceinline__ R drp_interior_unit(const F &f, int q)
+{
+    const R t1 = R(DRP_A1) * (f(q + 1) - f(q - 1));
+    const R t2 = R(DRP_A2) * (f(q + 2) - f(q - 2));
+    const R t3 = R(DRP_A3) * (f(q + 3) - f(q - 3));
+    return (t1 + t2) + t3;
+}
+
+// FAST: the tile and its 3-point level-1 halo lie in the interior, outside
+// the metric/buffer layers, on a unit grid.  Then no one-sided stencil, no
+// metric factor (it is exactly 1), no buffer and no edge test is needed, and
+// dropping the multiplications by 1.0 leaves every result bit unchanged.
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT, bool FAST>
+__device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int bx, int by)
 {
     constexpr int TX = STX;
     constexpr int WX = TX + 12, WY = TY + 12;
@@ -61,12 +76,20 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
     constexpr int LPSI = VISC ? 5 : 0;         // level-1 slot of the species flux
 
     const int imax = A.imax, jmax = A.jmax, pitch = A.pitch;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i0 = bx * TX, j0 = by * TY;
     const int i1 = min(i0 + TX, imax), j1 = min(j0 + TY, jmax);
     const Range rx = tile_ranges(i0, i1, imax);
     const Range ry = tile_ranges(j0, j1, jmax);
     const int tid = threadIdx.x;
     const R gam = R(A.gamma), inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn);
+    auto DV = [&](const auto &f, int q, int n, R inv_h) -> R {
+        if constexpr (FAST) return drp_interior_unit<R>(f, q);
+        else return drp_deriv<R>(f, q, n, inv_h);
+    };
+    auto MET = [&](R d, R m) -> R {
+        if constexpr (FAST) return d;
+        else return d * m;
+    };
 
     // phase 1: U window (and T = gamma p / rho, rhs.f90:20)
     const int wxn = rx.w1 - rx.w0, wyn = ry.w1 - ry.w0;
@@ -123,20 +146,20 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
             const R mx = R(A.metx[gx]), my = R(A.mety[gy]);
             const int o = ly * LX + lx;
             if constexpr (SPECIES) {
-                const R psx = drp_deriv<R>(UX(4, gy), gx, imax, inv_dm) * mx;
-                const R psy = drp_deriv<R>(UYc(4, gx), gy, jmax, inv_dn) * my + dbase(4, gy);
+                const R psx = MET(DV(UX(4, gy), gx, imax, inv_dm), mx);
+                const R psy = MET(DV(UYc(4, gx), gy, jmax, inv_dn), my) + dbase(4, gy);
                 const R rho = UX(0, gy)(gx) + base(0, gy);
                 sL[LPSI * LY * LX + o] = rho * R(A.alpha_cond) * (psx + psy);  // derivs.f90:40
             }
             if constexpr (!VISC) continue;
-            const R ux = drp_deriv<R>(UX(1, gy), gx, imax, inv_dm) * mx;
-            const R vx = drp_deriv<R>(UX(2, gy), gx, imax, inv_dm) * mx;
-            const R uy = drp_deriv<R>(UYc(1, gx), gy, jmax, inv_dn) * my + dbase(1, gy);
-            const R vy = drp_deriv<R>(UYc(2, gx), gy, jmax, inv_dn) * my + dbase(2, gy);
+            const R ux = MET(DV(UX(1, gy), gx, imax, inv_dm), mx);
+            const R vx = MET(DV(UX(2, gy), gx, imax, inv_dm), mx);
+            const R uy = MET(DV(UYc(1, gx), gy, jmax, inv_dn), my) + dbase(1, gy);
+            const R vy = MET(DV(UYc(2, gx), gy, jmax, inv_dn), my) + dbase(2, gy);
             const R *trow = sT + (gy - oy) * WX;
             const R *tcol = sT + (gx - ox);
-            const R Tx = drp_deriv<R>([trow, ox](int q) { return trow[q - ox]; }, gx, imax, inv_dm) * mx;
-            R Ty = drp_deriv<R>([tcol, oy](int q) { return tcol[(q - oy) * WX]; }, gy, jmax, inv_dn) * my;
+            const R Tx = MET(DV([trow, ox](int q) { return trow[q - ox]; }, gx, imax, inv_dm), mx);
+            R Ty = MET(DV([tcol, oy](int q) { return tcol[(q - oy) * WX]; }, gy, jmax, inv_dn), my);
             R Tval = trow[gx - ox];
             if constexpr (PERT) {
                 Ty += R(A.pr.dtb[gy]);
@@ -161,15 +184,15 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
     if (gx >= i1) return;
     const R mx = R(A.metx[gx]);
     for (int gy = j0 + (tid >> 6); gy < j1; gy += NT / 64) {
-        const bool interior = gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2;
+        const bool interior = FAST || (gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2);
         const R my = R(A.mety[gy]);
         R qc[NK], uc[NK], fx[NK], fy[NK];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
             qc[k] = UX(k, gy)(gx);
             uc[k] = qc[k] + base(k, gy);
-            fx[k] = drp_deriv<R>(UX(k, gy), gx, imax, inv_dm) * mx;
-            fy[k] = drp_deriv<R>(UYc(k, gx), gy, jmax, inv_dn) * my + dbase(k, gy);
+            fx[k] = MET(DV(UX(k, gy), gx, imax, inv_dm), mx);
+            fy[k] = MET(DV(UYc(k, gx), gy, jmax, inv_dn), my) + dbase(k, gy);
         }
         const R rho = uc[0], u = uc[1], v = uc[2], p = uc[3];
         const R irho = R(1) / rho;
@@ -187,12 +210,12 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
                 const R *col = sL + l * LY * LX + (gx - lx0);
                 return [col, ly0](int q) { return col[(q - ly0) * LX]; };
             };
-            const R txx_x = drp_deriv<R>(LXa(0), gx, imax, inv_dm) * mx;
-            const R txy_x = drp_deriv<R>(LXa(1), gx, imax, inv_dm) * mx;
-            const R ktx_x = drp_deriv<R>(LXa(3), gx, imax, inv_dm) * mx;
-            const R txy_y = drp_deriv<R>(LYa(1), gy, jmax, inv_dn) * my;
-            const R tyy_y = drp_deriv<R>(LYa(2), gy, jmax, inv_dn) * my;
-            const R kty_y = drp_deriv<R>(LYa(4), gy, jmax, inv_dn) * my;
+            const R txx_x = MET(DV(LXa(0), gx, imax, inv_dm), mx);
+            const R txy_x = MET(DV(LXa(1), gx, imax, inv_dm), mx);
+            const R ktx_x = MET(DV(LXa(3), gx, imax, inv_dm), mx);
+            const R txy_y = MET(DV(LYa(1), gy, jmax, inv_dn), my);
+            const R tyy_y = MET(DV(LYa(2), gy, jmax, inv_dn), my);
+            const R kty_y = MET(DV(LYa(4), gy, jmax, inv_dn), my);
             const int o = (gy - ly0) * LX + (gx - lx0);
             const R t0 = sL[0 * LY * LX + o], t1 = sL[1 * LY * LX + o], t2 = sL[2 * LY * LX + o];
             const R vre = R(A.inv_Re) * irho;
@@ -211,8 +234,8 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
             const int lx0 = rx.l0, ly0 = ry.l0;
             const R *prow = sL + (LPSI * LY + (gy - ly0)) * LX;
             const R *pcol = sL + LPSI * LY * LX + (gx - lx0);
-            const R psd_x = drp_deriv<R>([prow, lx0](int q) { return prow[q - lx0]; }, gx, imax, inv_dm) * mx;
-            const R psd_y = drp_deriv<R>([pcol, ly0](int q) { return pcol[(q - ly0) * LX]; }, gy, jmax, inv_dn) * my;
+            const R psd_x = MET(DV([prow, lx0](int q) { return prow[q - lx0]; }, gx, imax, inv_dm), mx);
+            const R psd_y = MET(DV([pcol, ly0](int q) { return pcol[(q - ly0) * LX]; }, gy, jmax, inv_dn), my);
             f[4] = -(u * fx[4] + v * fy[4] - R(A.inv_Pe) * irho * (psd_x + psd_y));  // eqeuler.f90:94-95
         }
         const size_t g = (size_t)gy * pitch + gx;
@@ -220,8 +243,8 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
 #pragma unroll
             for (int k = 0; k < NK; ++k) A.dudt[k][g] = S(f[k]);
         } else {
-            const bool ybuf = gy < A.D || gy >= jmax - A.D;
-            const bool xbuf = gx < A.D || gx >= imax - A.D;
+            const bool ybuf = !FAST && (gy < A.D || gy >= jmax - A.D);
+            const bool xbuf = !FAST && (gx < A.D || gx >= imax - A.D);
 #pragma unroll
             for (int k = 0; k < NK; ++k) {
                 if constexpr (PERT) {
@@ -255,6 +278,24 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A)
     }
 }
 
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT, bool FAST>
+__global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A, int bx0, int by0)
+{
+    stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, FAST>(A, bx0 + blockIdx.x, by0 + blockIdx.y);
+}
+
+// Batched launch: one block per entry of a tile table spanning every case;
+// args holds one StageArgs per (stage, case), so a captured graph replays the
+// whole batch without argument updates.  Fast and general tiles are listed in
+// separate tables and run as separate launches.
+template <typename S, typename R, int NK, bool VISC, int TY, bool PERT, int NT, bool FAST>
+__global__ __launch_bounds__(NT) void stage_kernel_batch(const StageArgs<S> *args, const int4 *tiles, int stage,
+                                                         int ncases)
+{
+    const int4 t = tiles[blockIdx.x];
+    stage_tile<S, R, NK, VISC, TY, false, PERT, NT, FAST>(args[(size_t)stage * ncases + t.x], t.y, t.z);
+}
+
 template <typename R>
 static size_t stage_smem(int NK, bool visc, int TY)
 {
@@ -263,18 +304,36 @@ static size_t stage_smem(int NK, bool visc, int TY)
     return sizeof(R) * ((size_t)NK * WY * WX + (visc ? (size_t)WY * WX : 0) + (size_t)NL * LY * LX);
 }
 
-template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT>
-static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s)
+template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, bool FAST>
+static hipError_t launch_stage_rect(const StageArgs<S> &A, int bx0, int bx1, int by0, int by1, hipStream_t s)
 {
     using R = S;
     constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
-    dim3 grid((A.imax + STX - 1) / STX, (A.jmax + TY - 1) / TY);
+    if (bx1 <= bx0 || by1 <= by0) return hipSuccess;
     size_t smem = stage_smem<R>(NK, VISC, TY);
-    auto kern = stage_kernel<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT>;
+    auto kern = stage_kernel<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, FAST>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, grid, dim3(NT), smem, s, A);
+    hipLaunchKernelGGL(kern, dim3(bx1 - bx0, by1 - by0), dim3(NT), smem, s, A, bx0, by0);
     return hipGetLastError();
+}
+
+// interior rectangle on the fast kernel, the ring of edge tiles on the general one
+template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT>
+static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s)
+{
+    constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
+    const int nbx = (A.imax + STX - 1) / STX, nby = (A.jmax + TY - 1) / TY;
+    int fx0, fx1, fy0, fy1;
+    fast_range(A.fast_ok, A.D, A.imax, STX, fx0, fx1);
+    fast_range(A.fast_ok, A.D, A.jmax, TY, fy0, fy1);
+    if (fx1 <= fx0 || fy1 <= fy0) return launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, nbx, 0, nby, s);
+    hipError_t e;
+    if ((e = launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, true>(A, fx0, fx1, fy0, fy1, s)) != hipSuccess) return e;
+    if ((e = launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, nbx, 0, fy0, s)) != hipSuccess) return e;
+    if ((e = launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, nbx, fy1, nby, s)) != hipSuccess) return e;
+    if ((e = launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, fx0, fy0, fy1, s)) != hipSuccess) return e;
+    return launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, fx1, nbx, fy0, fy1, s);
 }
 
 // block-size variants of the stage kernel (same tile, same arithmetic):
@@ -305,6 +364,67 @@ static hipError_t launch_stage_p(const StageArgs<S> &A, int nk, bool visc, bool 
     return rhs_only ? launch_stage_v<S, 5, false, true, PERT>(A, variant, s)
                     : launch_stage_v<S, 5, false, false, PERT>(A, variant, s);
 }
+
+template <typename S, int NK, bool VISC, bool PERT, int NT, bool FAST>
+static hipError_t launch_stage_batch_t(const StageArgs<S> *args, const int4 *tiles, int ntiles, int stage, int ncases,
+                                       hipStream_t s)
+{
+    using R = S;
+    constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
+    if (ntiles <= 0) return hipSuccess;
+    size_t smem = stage_smem<R>(NK, VISC, TY);
+    auto kern = stage_kernel_batch<S, R, NK, VISC, TY, PERT, NT, FAST>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(ntiles), dim3(NT), smem, s, args, tiles, stage, ncases);
+    return hipGetLastError();
+}
+
+template <typename S, bool PERT, int NK, bool VISC, int NT>
+static hipError_t launch_stage_batch_n(const StageArgs<S> *args, const int4 *ftiles, int nf, const int4 *gtiles,
+                                       int ng, int stage, int ncases, hipStream_t s)
+{
+    hipError_t e = launch_stage_batch_t<S, NK, VISC, PERT, NT, true>(args, ftiles, nf, stage, ncases, s);
+    if (e != hipSuccess) return e;
+    return launch_stage_batch_t<S, NK, VISC, PERT, NT, false>(args, gtiles, ng, stage, ncases, s);
+}
+
+template <typename S, bool PERT, int NK, bool VISC>
+static hipError_t launch_stage_batch_v(const StageArgs<S> *args, const int4 *ftiles, int nf, const int4 *gtiles,
+                                       int ng, int stage, int ncases, int variant, hipStream_t s)
+{
+    switch (variant) {
+    case 1: return launch_stage_batch_n<S, PERT, NK, VISC, 512>(args, ftiles, nf, gtiles, ng, stage, ncases, s);
+    case 2: return launch_stage_batch_n<S, PERT, NK, VISC, 1024>(args, ftiles, nf, gtiles, ng, stage, ncases, s);
+    default: return launch_stage_batch_n<S, PERT, NK, VISC, 256>(args, ftiles, nf, gtiles, ng, stage, ncases, s);
+    }
+}
+
+template <typename S, bool PERT>
+static hipError_t launch_stage_batch_p(const StageArgs<S> *args, const int4 *ftiles, int nf, const int4 *gtiles,
+                                       int ng, int stage, int ncases, int nk, bool visc, int variant, hipStream_t s)
+{
+    if (nk == 4)
+        return visc ? launch_stage_batch_v<S, PERT, 4, true>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s)
+                    : launch_stage_batch_v<S, PERT, 4, false>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s);
+    return visc ? launch_stage_batch_v<S, PERT, 5, true>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s)
+                : launch_stage_batch_v<S, PERT, 5, false>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s);
+}
+
+#if defined(NS_KERNELS_F64)
+hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
+                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s)
+{
+    return launch_stage_batch_p<double, false>(args, ftiles, nf, gtiles, ng, stage, ncases, nk, visc, variant, s);
+}
+#endif
+#if defined(NS_KERNELS_F32)
+hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
+                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s)
+{
+    return launch_stage_batch_p<float, true>(args, ftiles, nf, gtiles, ng, stage, ncases, nk, visc, variant, s);
+}
+#endif
 
 #if defined(NS_KERNELS_F64)
 hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s)
@@ -539,20 +659,19 @@ struct XIO {
 };
 
 template <typename R>
-__global__ __launch_bounds__(64) void filter_x_kernel(FilterArgs<R> F)
+__device__ __forceinline__ void filter_x_tile(const FilterArgs<R> &F, int chunk, int rowblock, int k)
 {
     __shared__ R tin[64][FX_T + 1];
     __shared__ R tout[64][FX_T + 1];
     int a, b;
-    if (!chunk_bounds(blockIdx.x, F.L, F.n, a, b)) return;
-    const int k = blockIdx.z;
+    if (!chunk_bounds(chunk, F.L, F.n, a, b)) return;
     XIO<R> io;
     io.in = F.in[k];
     io.out = F.out[k];
     io.pitch = F.pitch;
     io.n = F.n;
     io.m = F.m;
-    io.row0 = blockIdx.y * 64;
+    io.row0 = rowblock * 64;
     io.a = a;
     io.b = b;
     io.lane = threadIdx.x;
@@ -562,6 +681,22 @@ __global__ __launch_bounds__(64) void filter_x_kernel(FilterArgs<R> F)
     io.start(exact ? 0 : a - F.W - 6);
     LineEnds ends;
     filter_line<R, false>(io, R(F.sigma), F.n, a, b, F.W, ends);
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void filter_x_kernel(FilterArgs<R> F)
+{
+    filter_x_tile<R>(F, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// batched: tiles[blockIdx.x] = (case, chunk, row block), blockIdx.y = variable
+template <typename R>
+__global__ __launch_bounds__(64) void filter_x_kernel_batch(const FilterArgs<R> *args, const int4 *tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    const FilterArgs<R> &F = args[t.x];
+    if ((int)blockIdx.y >= F.nk) return;
+    filter_x_tile<R>(F, t.y, t.z, blockIdx.y);
 }
 
 // y direction: lanes are columns; rows are contiguous across lanes.
@@ -581,12 +716,11 @@ struct YIO {
 };
 
 template <typename R>
-__global__ __launch_bounds__(64) void filter_y_kernel(FilterArgs<R> F)
+__device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblock, int chunk, int k)
 {
     int a, b;
-    if (!chunk_bounds(blockIdx.y, F.L, F.n, a, b)) return;
-    const int k = blockIdx.z;
-    const int col = blockIdx.x * 64 + threadIdx.x;
+    if (!chunk_bounds(chunk, F.L, F.n, a, b)) return;
+    const int col = colblock * 64 + threadIdx.x;
     YIO<R> io;
     io.in = F.in[k];
     io.out = F.out[k];
@@ -627,6 +761,22 @@ __global__ __launch_bounds__(64) void filter_y_kernel(FilterArgs<R> F)
     }
 }
 
+template <typename R>
+__global__ __launch_bounds__(64) void filter_y_kernel(FilterArgs<R> F)
+{
+    filter_y_tile<R>(F, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// batched: tiles[blockIdx.x] = (case, column block, chunk), blockIdx.y = variable
+template <typename R>
+__global__ __launch_bounds__(64) void filter_y_kernel_batch(const FilterArgs<R> *args, const int4 *tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    const FilterArgs<R> &F = args[t.x];
+    if ((int)blockIdx.y >= F.nk) return;
+    filter_y_tile<R>(F, t.y, t.z, blockIdx.y);
+}
+
 template <typename S>
 static hipError_t launch_filter_t(const FilterArgs<S> &F, bool xdir, hipStream_t s)
 {
@@ -641,11 +791,32 @@ static hipError_t launch_filter_t(const FilterArgs<S> &F, bool xdir, hipStream_t
     return hipGetLastError();
 }
 
+template <typename S>
+static hipError_t launch_filter_batch_t(const FilterArgs<S> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
+                                        hipStream_t s)
+{
+    if (xdir)
+        hipLaunchKernelGGL(filter_x_kernel_batch<S>, dim3(ntiles, nk), dim3(64), 0, s, args, tiles);
+    else
+        hipLaunchKernelGGL(filter_y_kernel_batch<S>, dim3(ntiles, nk), dim3(64), 0, s, args, tiles);
+    return hipGetLastError();
+}
+
 #if defined(NS_KERNELS_F64)
 hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s) { return launch_filter_t(F, xdir, s); }
+hipError_t launch_filter_batch(const FilterArgs<double> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
+                               hipStream_t s)
+{
+    return launch_filter_batch_t(args, tiles, ntiles, nk, xdir, s);
+}
 #endif
 #if defined(NS_KERNELS_F32)
 hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s) { return launch_filter_t(F, xdir, s); }
+hipError_t launch_filter_batch(const FilterArgs<float> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
+                               hipStream_t s)
+{
+    return launch_filter_batch_t(args, tiles, ntiles, nk, xdir, s);
+}
 #endif
 
 // ---------------------------------------------------------------------------
@@ -653,10 +824,8 @@ hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s) {
 // corner rows j=1, jmax computed from those columns' values (cc.f90:23-59).
 // ---------------------------------------------------------------------------
 template <typename S>
-__global__ void edge_i_kernel(EdgeArgs<S> E)
+__device__ __forceinline__ void edge_i_point(const EdgeArgs<S> &E, int j, int side)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int side = blockIdx.y;  // 0: inflow, 1: outflow
     if (j >= E.jmax) return;
     const int imax = E.imax, jmax = E.jmax, pitch = E.pitch;
     const double ig = 1.0 / E.gamma;
@@ -694,6 +863,21 @@ __global__ void edge_i_kernel(EdgeArgs<S> E)
 }
 
 template <typename S>
+__global__ void edge_i_kernel(EdgeArgs<S> E)
+{
+    // blockIdx.y: 0 inflow column, 1 outflow column
+    edge_i_point<S>(E, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+// batched: blockIdx.x = case, threads stride over the rows
+template <typename S>
+__global__ void edge_i_kernel_batch(const EdgeArgs<S> *args)
+{
+    const EdgeArgs<S> &E = args[blockIdx.x];
+    for (int j = threadIdx.x; j < E.jmax; j += blockDim.x) edge_i_point<S>(E, j, blockIdx.y);
+}
+
+template <typename S>
 static hipError_t launch_edge_t(const EdgeArgs<S> &E, hipStream_t s)
 {
     dim3 grid((E.jmax + 255) / 256, 2);
@@ -701,11 +885,26 @@ static hipError_t launch_edge_t(const EdgeArgs<S> &E, hipStream_t s)
     return hipGetLastError();
 }
 
+template <typename S>
+static hipError_t launch_edge_batch_t(const EdgeArgs<S> *args, int ncases, hipStream_t s)
+{
+    hipLaunchKernelGGL(edge_i_kernel_batch<S>, dim3(ncases, 2), dim3(256), 0, s, args);
+    return hipGetLastError();
+}
+
 #if defined(NS_KERNELS_F64)
 hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s) { return launch_edge_t(E, s); }
+hipError_t launch_edge_batch(const EdgeArgs<double> *args, int ncases, hipStream_t s)
+{
+    return launch_edge_batch_t(args, ncases, s);
+}
 #endif
 #if defined(NS_KERNELS_F32)
 hipError_t launch_edge_i(const EdgeArgs<float> &E, hipStream_t s) { return launch_edge_t(E, s); }
+hipError_t launch_edge_batch(const EdgeArgs<float> *args, int ncases, hipStream_t s)
+{
+    return launch_edge_batch_t(args, ncases, s);
+}
 #endif
 
 #if defined(NS_KERNELS_F64)

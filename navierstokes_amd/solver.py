@@ -34,36 +34,48 @@ def _f64(a, shape=None):
     return a
 
 
+def make_params(case: Case, device: int = 0, precision: int = _lib.NS_PREC_FP64) -> _lib.NSParams:
+    """ns_params of a case (the reference's `module global` scalars)."""
+    lib = _lib.load()
+    p = _lib.NSParams()
+    _lib.check(lib.ns_default_params(C.byref(p)))
+    tc = case.transport()
+    p.imax, p.jmax, p.nk, p.D = case.imax, case.jmax, case.nk, case.D
+    p.dm, p.dn, p.dt, p.sigmaf = case.dm, case.dn, case.dt, case.sigmaf
+    p.gamma, p.Re, p.Pr, p.Pe = case.gamma, case.Re, case.Pr, case.Pe
+    p.alpha_cond, p.Tref = case.alpha_cond, case.Tref
+    for key in ("A_mi", "B_mi", "C_mi", "D_mi", "MI0", "A_k", "B_k", "C_k", "D_k", "K0"):
+        setattr(p, key, tc[key])
+    p.amplitude, p.wf, p.r0_acous = case.amplitude, case.wf, case.r0_acous
+    p.m_pt_acous, p.n_pt_acous = case.m_pt_acous, case.n_pt_acous
+    p.precision = precision
+    flags = 0
+    if case.inviscid:
+        flags |= _lib.NS_FLAG_INVISCID
+    if case.metric_everywhere:
+        flags |= _lib.NS_FLAG_METRIC_EVERYWHERE
+    p.flags = flags
+    p.device = device
+    return p
+
+
+def _profiles(case: Case):
+    c = case
+    return [_f64(x) for x in (c.meshx, c.meshy, c.sigmax, c.sigmay, c.m, c.n, c.rhob, c.psib)]
+
+
 class Solver:
     def __init__(self, case: Case, device: int = 0, precision: int = _lib.NS_PREC_FP64):
         self.case = case
         self.lib = _lib.load()
-        p = _lib.NSParams()
-        _lib.check(self.lib.ns_default_params(C.byref(p)))
-        tc = case.transport()
-        p.imax, p.jmax, p.nk, p.D = case.imax, case.jmax, case.nk, case.D
-        p.dm, p.dn, p.dt, p.sigmaf = case.dm, case.dn, case.dt, case.sigmaf
-        p.gamma, p.Re, p.Pr, p.Pe = case.gamma, case.Re, case.Pr, case.Pe
-        p.alpha_cond, p.Tref = case.alpha_cond, case.Tref
-        for key in ("A_mi", "B_mi", "C_mi", "D_mi", "MI0", "A_k", "B_k", "C_k", "D_k", "K0"):
-            setattr(p, key, tc[key])
-        p.amplitude, p.wf, p.r0_acous = case.amplitude, case.wf, case.r0_acous
-        p.m_pt_acous, p.n_pt_acous = case.m_pt_acous, case.n_pt_acous
-        p.precision = precision
-        flags = 0
-        if case.inviscid:
-            flags |= _lib.NS_FLAG_INVISCID
-        if case.metric_everywhere:
-            flags |= _lib.NS_FLAG_METRIC_EVERYWHERE
-        p.flags = flags
-        p.device = device
+        p = make_params(case, device, precision)
         self.params = p
         ctx = C.c_void_p()
         _lib.check(self.lib.ns_create(C.byref(p), C.byref(ctx)))
         self.ctx = ctx
         self.shape = (case.jmax, case.imax, case.nk)
         c = case
-        self._prof = [_f64(x) for x in (c.meshx, c.meshy, c.sigmax, c.sigmay, c.m, c.n, c.rhob, c.psib)]
+        self._prof = _profiles(c)
         _lib.check(self.lib.ns_set_profiles(self.ctx, *[_lib.dptr(x) for x in self._prof]))
         self.set_base(c.Ub)
         self.put_state(c.U0)
@@ -194,3 +206,60 @@ class Solver:
         cnt = C.c_int32()
         _lib.check(self.lib.ns_kernel_time(self.ctx, name.encode(), C.byref(avg), C.byref(cnt)))
         return avg.value, cnt.value
+
+
+class BatchSolver:
+    """Many independent cases (any grid sizes) advanced together.
+
+    The reference runs one case per process (euler.f90:123-170 is the whole
+    time loop); here every kernel class of a step runs as one launch over all
+    cases and a captured HIP graph replays the batch step.  All cases share
+    nk, precision and flags, and run without the acoustic source.
+    """
+
+    def __init__(self, cases, device: int = 0, precision: int = _lib.NS_PREC_FP64):
+        self.cases = list(cases)
+        self.lib = _lib.load()
+        n = len(self.cases)
+        arr = (_lib.NSParams * n)(*[make_params(c, device, precision) for c in self.cases])
+        b = C.c_void_p()
+        _lib.check(self.lib.ns_batch_create(arr, n, C.byref(b)))
+        self.b = b
+        for i, c in enumerate(self.cases):
+            shape = (c.jmax, c.imax, c.nk)
+            prof = _profiles(c)
+            Ub, U0 = _f64(c.Ub, shape), _f64(c.U0, shape)
+            _lib.check(self.lib.ns_batch_set_case(self.b, i, *[_lib.dptr(x) for x in prof],
+                                                  _lib.dptr(Ub), _lib.dptr(U0)))
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.ns_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, iter0: int, nsteps: int):
+        _lib.check(self.lib.ns_batch_steps(self.b, int(iter0), int(nsteps)))
+
+    def get_state(self, c: int) -> np.ndarray:
+        cs = self.cases[c]
+        U = np.empty((cs.jmax, cs.imax, cs.nk))
+        _lib.check(self.lib.ns_batch_get_state(self.b, int(c), _lib.dptr(U)))
+        return U
+
+    def autotune(self):
+        v = C.c_int32()
+        ms = np.zeros(3)
+        _lib.check(self.lib.ns_batch_autotune(self.b, C.byref(v), _lib.dptr(ms)))
+        return v.value, ms

@@ -1,0 +1,100 @@
+"""Batched cases: every case of a batch advances exactly as it would alone.
+
+The batch runs each kernel class as one launch over a tile table spanning all
+cases (and replays a captured graph per step), but each tile computes with its
+own case's arguments, so the results must be bitwise those of a single-case
+context on the same inputs, in both precisions."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from navierstokes_amd import config
+
+pytestmark = pytest.mark.gpu
+
+FP64, FP32P = 0, 1
+
+
+def _uneven_pulses(seed):
+    return config.batch_sweep_cases(5, seed=seed, sizes=(64, 96, 128, 160), D=12)
+
+
+def _singles(cases, prec, nsteps):
+    from navierstokes_amd.solver import Solver
+    out = []
+    for c in cases:
+        s = Solver(c, precision=prec)
+        s.run(0, nsteps)
+        out.append(s.get_state())
+        s.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32P])
+def test_batch_equals_single_euler(prec):
+    from navierstokes_amd.solver import BatchSolver
+    cases = _uneven_pulses(41)
+    print(f"[case] batch of {[c.imax for c in cases]} precision={prec}", flush=True)
+    ref = _singles(cases, prec, 12)
+    b = BatchSolver(cases, precision=prec)
+    b.run(0, 5)
+    b.run(5, 7)
+    for i, c in enumerate(cases):
+        got = b.get_state(i)
+        assert np.array_equal(got, ref[i]), f"case {i} ({c.name}): max diff {np.abs(got - ref[i]).max():.3e}"
+    b.close()
+
+
+@pytest.mark.parametrize("nk", [4, 5])
+def test_batch_equals_single_viscous(nk):
+    from navierstokes_amd.solver import BatchSolver
+    cases = [config.random_case(n, m, nk=nk, D=8, seed=50 + n, amp=1e-3, mesh_random=(n % 2 == 0))
+             for n, m in ((80, 72), (133, 64), (64, 150))]
+    print(f"[case] viscous batch nk={nk}", flush=True)
+    for prec in (FP64, FP32P):
+        ref = _singles(cases, prec, 6)
+        b = BatchSolver(cases, precision=prec)
+        b.run(0, 6)
+        for i in range(len(cases)):
+            assert np.array_equal(b.get_state(i), ref[i]), f"prec {prec} case {i}"
+        b.close()
+
+
+def test_batch_autotune_is_bitwise_neutral():
+    from navierstokes_amd.solver import BatchSolver
+    cases = _uneven_pulses(43)
+    b0 = BatchSolver(cases, precision=FP32P)
+    b0.run(0, 4)
+    b1 = BatchSolver(cases, precision=FP32P)
+    v, ms = b1.autotune()
+    print(f"batch autotune: variant {v}, ms {ms}", flush=True)
+    assert v in (0, 1, 2)
+    b1.run(0, 4)
+    for i in range(len(cases)):
+        assert np.array_equal(b0.get_state(i), b1.get_state(i))
+    b0.close()
+    b1.close()
+
+
+def test_batch_fp32_perturbation_error():
+    from navierstokes_amd.solver import BatchSolver
+    cases = _uneven_pulses(44)
+    b32 = BatchSolver(cases, precision=FP32P)
+    b64 = BatchSolver(cases, precision=FP64)
+    b32.run(0, 200)
+    b64.run(0, 200)
+    for i, c in enumerate(cases):
+        a, r = b32.get_state(i), b64.get_state(i)
+        err = rel_l2(a[..., 3] - c.Ub[..., 3], r[..., 3] - c.Ub[..., 3])
+        print(f"case {i} {c.name}: p' relative L2 {err:.3e}", flush=True)
+        assert err < 1e-4
+    b32.close()
+    b64.close()
+
+
+def test_batch_rejects_mixed_nk():
+    from navierstokes_amd._lib import NSError
+    from navierstokes_amd.solver import BatchSolver
+    cases = [config.acoustic_pulse_case(64, D=8), config.random_case(64, 64, nk=5, D=8, seed=1)]
+    with pytest.raises(NSError):
+        BatchSolver(cases)
