@@ -6,6 +6,13 @@ flow M=0.5, Gaussian acoustic pulse + isentropic vortex, 13-point filter
 sigma=0.2 in every stage, 40-point buffer zones, FP64.  A "step" is one full
 LDDRK6 step (6 stages of RHS + update + buffer + x/y filter + edges).
 
+`--workload batch`: the batched sweep -- 256 independent 2-D Euler Gaussian
+pulse cases on square grids drawn from {128,192,256,384,512}^2 (seeded; M in
+[0,0.6], amplitude log-uniform in [1e-4,1e-2] p0, half-width 2-6), advanced
+together by the batched path (one launch per kernel class, graph per step),
+FP32-perturbation form by default; reports the aggregate rate and the p'
+error against the FP64 batch after the same steps.
+
 Prints one JSON line: Mpts*steps/s over the timed steps, the roofline of the
 dominant kernel (stage kernel, HBM bound) from HIP events on the library's
 stream, the whole-step roofline against the 704 B/pt/step compulsory
@@ -73,17 +80,98 @@ def copy_probe(mib: int):
     print(json.dumps({"copy_probe_MiB": mib, "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
 
 
+def bench_batch(args):
+    from navierstokes_amd import config
+    from navierstokes_amd.solver import BatchSolver
+
+    args.precision = args.precision or "fp32pert"
+    prec = 1 if args.precision == "fp32pert" else 0
+    cases = config.batch_sweep_cases(args.ncases, seed=2024)
+    npts = sum(c.npoints for c in cases)
+    sizes = sorted({c.imax for c in cases})
+    desc = (f"euler-pulse batched sweep: {len(cases)} cases, square grids {sizes}, "
+            f"{npts / 1e6:.2f} Mpts, M~U[0,0.6], amp~logU[1e-4,1e-2], hw~U[2,6], D=20, "
+            f"{'fp32-perturbation' if prec else 'fp64'} lddrk6 sigma=0.2")
+    print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
+    b = BatchSolver(cases, precision=prec)
+    variant, tune_ms = b.autotune()
+    b.run(0, args.warmup)
+    print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b.run(args.warmup, args.steps)
+    t = time.perf_counter() - t0
+    ms_step = t / args.steps * 1e3
+    value = npts * args.steps / t / 1e6
+    nk, elem = cases[0].nk, (4 if prec else 8)
+    B_step = alg_bytes_per_point_step(nk, elem)
+    ach = B_step * npts / (ms_step * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpts·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec else "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "ncases": len(cases), "points": npts, "nk": nk,
+                   "precision": args.precision, "parallelism": "single-gpu",
+                   "stage_block_variant": variant, "stage_variant_ms": [round(x, 4) for x in tune_ms]},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole batched step (graph of stage, filter and edge launches)",
+                     "alg_bytes_per_launch": B_step * npts, "avg_launch_ms": round(ms_step, 4)},
+    }
+    if prec:
+        # p' error of every case against the FP64 batch (bitwise the FP64 oracle)
+        nsteps = args.warmup + args.steps
+        ref = BatchSolver(cases, precision=0)
+        ref.run(0, nsteps)
+        errs = []
+        for i, c in enumerate(cases):
+            a, r = b.get_state(i), ref.get_state(i)
+            pa, pr = a[..., 3] - c.Ub[..., 3], r[..., 3] - c.Ub[..., 3]
+            errs.append(float(np.linalg.norm(pa - pr) / np.linalg.norm(pr)))
+        ref.close()
+        out["pprime_rel_l2_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)),
+                                        "steps": nsteps}
+    b.close()
+    if not args.no_cpu_baseline:
+        from oracle.oracle import Oracle
+        sample = cases[:16]
+        t0 = time.perf_counter()
+        threads = 1
+        for c in sample:
+            o = Oracle(c, parallel=True)
+            threads = o.threads()
+            o.run(c.U0, 0, args.cpu_steps)
+        dt = time.perf_counter() - t0
+        spts = sum(c.npoints for c in sample)
+        out["cpu_baseline"] = {"value": spts * args.cpu_steps / dt / 1e6, "unit": "Mpts·steps/s",
+                               "cores": int(threads), "kind": "port",
+                               "sample": f"{args.cpu_steps} LDDRK6 step(s) of the first {len(sample)} cases "
+                                         f"({spts / 1e6:.2f} Mpts), OpenMP build of oracle/ns_oracle.c "
+                                         f"({dt:.1f} s incl. per-case set-up)"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["ns", "pulse"], default="ns")
+    ap.add_argument("--workload", choices=["ns", "pulse", "batch"], default="ns")
+    ap.add_argument("--ncases", type=int, default=256)
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=3)
-    ap.add_argument("--precision", choices=["fp64", "fp32pert"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32pert"], default=None,
+                    help="default fp64 (ns, pulse) / fp32pert (batch)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -94,6 +182,11 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        bench_batch(args)
+        return
+    args.precision = args.precision or "fp64"
 
     from navierstokes_amd.solver import Solver
 

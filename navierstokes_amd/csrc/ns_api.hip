@@ -20,16 +20,18 @@
 #include "ns_kernels.h"
 
 namespace nsamd {
-hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s);
-hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s);
+hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
+                        hipStream_t s2);
+hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
+                        hipStream_t s2);
 hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s);
 hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<float> &E, hipStream_t s);
 hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
-                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s);
+                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
 hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
-                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s);
+                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
 hipError_t launch_filter_batch(const FilterArgs<double> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
                                hipStream_t s);
 hipError_t launch_filter_batch(const FilterArgs<float> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
@@ -87,6 +89,8 @@ struct ns_ctx {
     size_t plane = 0;  // elements per plane
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t side = nullptr;  // edge-ring stage tiles run here, concurrent with the interior
+    hipEvent_t fork = nullptr, join = nullptr;
     bool pert = false;        // FP32 perturbation form
     size_t esz = 8;           // bytes per state element
     void *S[3] = {nullptr, nullptr, nullptr};  // state buffers (FP64 U or FP32 q)
@@ -314,7 +318,13 @@ int enqueue_step_t(ns_ctx *c, int iter, bool timed)
         };
         StageArgs<S> A = stage_args<S>(c, irk, S0, S1, stage_amp(P, irk, iter), pert);
         tick(0);
-        HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream));
+        // interior tiles on the context's stream, edge-ring tiles on the side
+        // stream (fork after the previous work, join before the filters)
+        HIPCHK(hipEventRecord(c->fork, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->side, c->fork, 0));
+        HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream, c->side));
+        HIPCHK(hipEventRecord(c->join, c->side));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->join, 0));
         tick(1);
         HIPCHK(launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, c->stream));
         tick(2);
@@ -354,14 +364,14 @@ int autotune_stage_t(ns_ctx *c)
     double best_ms = 1e300;
     const int reps = 3;
     for (int v = 0; v < 3; ++v) {
-        hipError_t e = launch_stage(A, P.nk, visc, false, v, c->stream);
+        hipError_t e = launch_stage(A, P.nk, visc, false, v, c->stream, nullptr);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             c->tune_ms[v] = -1.0;
             continue;
         }
         HIPCHK(hipEventRecord(c->ev[0], c->stream));
-        for (int r = 0; r < reps; ++r) HIPCHK(launch_stage(A, P.nk, visc, false, v, c->stream));
+        for (int r = 0; r < reps; ++r) HIPCHK(launch_stage(A, P.nk, visc, false, v, c->stream, nullptr));
         HIPCHK(hipEventRecord(c->ev[1], c->stream));
         HIPCHK(hipEventSynchronize(c->ev[1]));
         float ms = 0.f;
@@ -550,6 +560,9 @@ void release(ns_ctx *c)
     for (auto &e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    if (c->fork) hipEventDestroy(c->fork);
+    if (c->join) hipEventDestroy(c->join);
+    if (c->side) hipStreamDestroy(c->side);
 }
 
 }  // namespace
@@ -643,6 +656,10 @@ int32_t ns_create(const ns_params *p, ns_ctx **out)
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
     c->own_stream = true;
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+    if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "side stream/events: %s", hipGetErrorString(e)));
     const size_t sbytes = (size_t)P.nk * c->plane * c->esz;
     for (void **pp : {&c->S[0], &c->S[1], &c->S[2], &c->Wr}) {
         e = hipMalloc(pp, sbytes);
@@ -925,7 +942,7 @@ int32_t ns_rhs(ns_ctx *c, double *dUdt, const double *U, double ct, int32_t iter
     StageArgs<double> A = stage_args<double>(c, 0, in, outb, amp, false);
     for (int k = 0; k < P.nk; ++k) A.dudt[k] = plane_ptr(outb, c, k);
     HIPCHK(launch_stage(A, P.nk, !(P.flags & NS_FLAG_INVISCID), true, c->stage_variant < 0 ? 0 : c->stage_variant,
-                        c->stream));
+                        c->stream, nullptr));
     return download_aos(c, outb, dUdt, P.nk);
 }
 
@@ -1056,6 +1073,8 @@ struct ns_batch {
     int n = 0, nk = 0;
     bool pert = false, visc = true;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // general (edge) stage tiles, concurrent with the fast ones
+    hipEvent_t fork = nullptr, join = nullptr;
     void *sargs = nullptr, *fxargs = nullptr, *fyargs = nullptr, *eargs = nullptr;
     int4 *stiles = nullptr, *gtiles = nullptr, *fxtiles = nullptr, *fytiles = nullptr;
     int nst = 0, ngt = 0, nfx = 0, nfy = 0;
@@ -1077,6 +1096,9 @@ void batch_release(ns_batch *b)
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     for (ns_ctx *c : b->cs)
         if (c) {
             release(c);
@@ -1143,8 +1165,12 @@ template <typename S>
 int batch_enqueue_step_t(ns_batch *b)
 {
     for (int irk = 0; irk < 6; ++irk) {
+        HIPCHK(hipEventRecord(b->fork, b->stream));
+        HIPCHK(hipStreamWaitEvent(b->side, b->fork, 0));
         HIPCHK(launch_stage_batch(static_cast<const StageArgs<S> *>(b->sargs), b->stiles, b->nst, b->gtiles, b->ngt, irk,
-                                  b->n, b->nk, b->visc, b->variant < 0 ? 0 : b->variant, b->stream));
+                                  b->n, b->nk, b->visc, b->variant < 0 ? 0 : b->variant, b->stream, b->side));
+        HIPCHK(hipEventRecord(b->join, b->side));
+        HIPCHK(hipStreamWaitEvent(b->stream, b->join, 0));
         HIPCHK(launch_filter_batch(static_cast<const FilterArgs<S> *>(b->fxargs), b->fxtiles, b->nfx, b->nk, true,
                                    b->stream));
         HIPCHK(launch_filter_batch(static_cast<const FilterArgs<S> *>(b->fyargs), b->fytiles, b->nfy, b->nk, false,
@@ -1167,7 +1193,8 @@ int batch_autotune_t(ns_batch *b)
     double best_ms = 1e300;
     for (int v = 0; v < 3; ++v) {
         const StageArgs<S> *sa = static_cast<const StageArgs<S> *>(b->sargs);
-        hipError_t e = launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 0, b->n, b->nk, b->visc, v, b->stream);
+        hipError_t e =
+            launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 0, b->n, b->nk, b->visc, v, b->stream, nullptr);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             b->tune_ms[v] = -1.0;
@@ -1175,7 +1202,8 @@ int batch_autotune_t(ns_batch *b)
         }
         HIPCHK(hipEventRecord(b->ev0, b->stream));
         for (int r = 0; r < 3; ++r)
-            HIPCHK(launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 0, b->n, b->nk, b->visc, v, b->stream));
+            HIPCHK(launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 0, b->n, b->nk, b->visc, v, b->stream,
+                                      nullptr));
         HIPCHK(hipEventRecord(b->ev1, b->stream));
         HIPCHK(hipEventSynchronize(b->ev1));
         float ms = 0.f;
@@ -1233,6 +1261,9 @@ int32_t ns_batch_create(const ns_params *params, int32_t ncases, ns_batch **out)
         }
     }
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e != hipSuccess) {

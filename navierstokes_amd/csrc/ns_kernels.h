@@ -92,4 +92,37 @@ __host__ __device__ inline void fast_range(int fast_ok, int D, int n, int T, int
     if (!fast_ok || n - Dm - 3 - T < 0 || b1 < b0) b0 = b1 = 0;
 }
 
+// The tiles of an nbx x nby tile grid outside the fast rectangle
+// [fx0,fx1) x [fy0,fy1), enumerated as bottom band, top band, then the left
+// and right strips row by row.
+struct RingMap {
+    int nbx, nby, fx0, fx1, fy0, fy1;
+    __host__ __device__ int count() const
+    {
+        return (fy0 + nby - fy1) * nbx + (fy1 - fy0) * (fx0 + nbx - fx1);
+    }
+    __host__ __device__ void tile(int t, int &bx, int &by) const
+    {
+        const int nb = fy0 * nbx;
+        if (t < nb) {
+            by = t / nbx;
+            bx = t - by * nbx;
+            return;
+        }
+        t -= nb;
+        const int nt = (nby - fy1) * nbx;
+        if (t < nt) {
+            const int r = t / nbx;
+            by = fy1 + r;
+            bx = t - r * nbx;
+            return;
+        }
+        t -= nt;
+        const int wl = fx0, w = fx0 + nbx - fx1;
+        const int r = t / w, c = t - r * w;
+        by = fy0 + r;
+        bx = c < wl ? c : fx1 + (c - wl);
+    }
+};
+
 }  // namespace nsamd

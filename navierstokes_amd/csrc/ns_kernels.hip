@@ -284,6 +284,14 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A, int bx0, int 
     stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, FAST>(A, bx0 + blockIdx.x, by0 + blockIdx.y);
 }
 
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT>
+__global__ __launch_bounds__(NT) void stage_kernel_ring(StageArgs<S> A, RingMap m)
+{
+    int bx, by;
+    m.tile(blockIdx.x, bx, by);
+    stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, false>(A, bx, by);
+}
+
 // Batched launch: one block per entry of a tile table spanning every case;
 // args holds one StageArgs per (stage, case), so a captured graph replays the
 // whole batch without argument updates.  Fast and general tiles are listed in
@@ -318,51 +326,57 @@ static hipError_t launch_stage_rect(const StageArgs<S> &A, int bx0, int bx1, int
     return hipGetLastError();
 }
 
-// interior rectangle on the fast kernel, the ring of edge tiles on the general one
+// interior rectangle on the fast kernel (stream s), the ring of edge tiles on
+// the general one in a single launch (stream s2 when given, so the two run
+// concurrently; the caller orders s2 after s and joins it back)
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT>
-static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s)
+static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream_t s2)
 {
+    using R = S;
     constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
     const int nbx = (A.imax + STX - 1) / STX, nby = (A.jmax + TY - 1) / TY;
     int fx0, fx1, fy0, fy1;
     fast_range(A.fast_ok, A.D, A.imax, STX, fx0, fx1);
     fast_range(A.fast_ok, A.D, A.jmax, TY, fy0, fy1);
     if (fx1 <= fx0 || fy1 <= fy0) return launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, nbx, 0, nby, s);
-    hipError_t e;
-    if ((e = launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, true>(A, fx0, fx1, fy0, fy1, s)) != hipSuccess) return e;
-    if ((e = launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, nbx, 0, fy0, s)) != hipSuccess) return e;
-    if ((e = launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, nbx, fy1, nby, s)) != hipSuccess) return e;
-    if ((e = launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, fx0, fy0, fy1, s)) != hipSuccess) return e;
-    return launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, fx1, nbx, fy0, fy1, s);
+    const RingMap m{nbx, nby, fx0, fx1, fy0, fy1};
+    const size_t smem = stage_smem<R>(NK, VISC, TY);
+    auto ring = stage_kernel_ring<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT>;
+    hipError_t e = hipFuncSetAttribute((const void *)ring, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ring, dim3(m.count()), dim3(NT), smem, s2 ? s2 : s, A, m);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, true>(A, fx0, fx1, fy0, fy1, s);
 }
 
 // block-size variants of the stage kernel (same tile, same arithmetic):
 // 0 -> 256 threads, 1 -> 512, 2 -> 1024
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
-static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s)
+static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s, hipStream_t s2)
 {
     switch (variant) {
-    case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512>(A, s);
-    case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 1024>(A, s);
-    default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 256>(A, s);
+    case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512>(A, s, s2);
+    case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 1024>(A, s, s2);
+    default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 256>(A, s, s2);
     }
 }
 
 template <typename S, bool PERT>
-static hipError_t launch_stage_p(const StageArgs<S> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s)
+static hipError_t launch_stage_p(const StageArgs<S> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
+                                 hipStream_t s2)
 {
     if (nk == 4) {
         if (visc)
-            return rhs_only ? launch_stage_v<S, 4, true, true, PERT>(A, variant, s)
-                            : launch_stage_v<S, 4, true, false, PERT>(A, variant, s);
-        return rhs_only ? launch_stage_v<S, 4, false, true, PERT>(A, variant, s)
-                        : launch_stage_v<S, 4, false, false, PERT>(A, variant, s);
+            return rhs_only ? launch_stage_v<S, 4, true, true, PERT>(A, variant, s, s2)
+                            : launch_stage_v<S, 4, true, false, PERT>(A, variant, s, s2);
+        return rhs_only ? launch_stage_v<S, 4, false, true, PERT>(A, variant, s, s2)
+                        : launch_stage_v<S, 4, false, false, PERT>(A, variant, s, s2);
     }
     if (visc)
-        return rhs_only ? launch_stage_v<S, 5, true, true, PERT>(A, variant, s)
-                        : launch_stage_v<S, 5, true, false, PERT>(A, variant, s);
-    return rhs_only ? launch_stage_v<S, 5, false, true, PERT>(A, variant, s)
-                    : launch_stage_v<S, 5, false, false, PERT>(A, variant, s);
+        return rhs_only ? launch_stage_v<S, 5, true, true, PERT>(A, variant, s, s2)
+                        : launch_stage_v<S, 5, true, false, PERT>(A, variant, s, s2);
+    return rhs_only ? launch_stage_v<S, 5, false, true, PERT>(A, variant, s, s2)
+                    : launch_stage_v<S, 5, false, false, PERT>(A, variant, s, s2);
 }
 
 template <typename S, int NK, bool VISC, bool PERT, int NT, bool FAST>
@@ -382,61 +396,67 @@ static hipError_t launch_stage_batch_t(const StageArgs<S> *args, const int4 *til
 
 template <typename S, bool PERT, int NK, bool VISC, int NT>
 static hipError_t launch_stage_batch_n(const StageArgs<S> *args, const int4 *ftiles, int nf, const int4 *gtiles,
-                                       int ng, int stage, int ncases, hipStream_t s)
+                                       int ng, int stage, int ncases, hipStream_t s, hipStream_t s2)
 {
-    hipError_t e = launch_stage_batch_t<S, NK, VISC, PERT, NT, true>(args, ftiles, nf, stage, ncases, s);
+    hipError_t e = launch_stage_batch_t<S, NK, VISC, PERT, NT, false>(args, gtiles, ng, stage, ncases, s2 ? s2 : s);
     if (e != hipSuccess) return e;
-    return launch_stage_batch_t<S, NK, VISC, PERT, NT, false>(args, gtiles, ng, stage, ncases, s);
+    return launch_stage_batch_t<S, NK, VISC, PERT, NT, true>(args, ftiles, nf, stage, ncases, s);
 }
 
 template <typename S, bool PERT, int NK, bool VISC>
 static hipError_t launch_stage_batch_v(const StageArgs<S> *args, const int4 *ftiles, int nf, const int4 *gtiles,
-                                       int ng, int stage, int ncases, int variant, hipStream_t s)
+                                       int ng, int stage, int ncases, int variant, hipStream_t s,
+                                       hipStream_t s2)
 {
     switch (variant) {
-    case 1: return launch_stage_batch_n<S, PERT, NK, VISC, 512>(args, ftiles, nf, gtiles, ng, stage, ncases, s);
-    case 2: return launch_stage_batch_n<S, PERT, NK, VISC, 1024>(args, ftiles, nf, gtiles, ng, stage, ncases, s);
-    default: return launch_stage_batch_n<S, PERT, NK, VISC, 256>(args, ftiles, nf, gtiles, ng, stage, ncases, s);
+    case 1: return launch_stage_batch_n<S, PERT, NK, VISC, 512>(args, ftiles, nf, gtiles, ng, stage, ncases, s, s2);
+    case 2: return launch_stage_batch_n<S, PERT, NK, VISC, 1024>(args, ftiles, nf, gtiles, ng, stage, ncases, s, s2);
+    default: return launch_stage_batch_n<S, PERT, NK, VISC, 256>(args, ftiles, nf, gtiles, ng, stage, ncases, s, s2);
     }
 }
 
 template <typename S, bool PERT>
 static hipError_t launch_stage_batch_p(const StageArgs<S> *args, const int4 *ftiles, int nf, const int4 *gtiles,
-                                       int ng, int stage, int ncases, int nk, bool visc, int variant, hipStream_t s)
+                                       int ng, int stage, int ncases, int nk, bool visc, int variant, hipStream_t s,
+                                       hipStream_t s2)
 {
     if (nk == 4)
-        return visc ? launch_stage_batch_v<S, PERT, 4, true>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s)
-                    : launch_stage_batch_v<S, PERT, 4, false>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s);
-    return visc ? launch_stage_batch_v<S, PERT, 5, true>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s)
-                : launch_stage_batch_v<S, PERT, 5, false>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s);
+        return visc ? launch_stage_batch_v<S, PERT, 4, true>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s, s2)
+                    : launch_stage_batch_v<S, PERT, 4, false>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s, s2);
+    return visc ? launch_stage_batch_v<S, PERT, 5, true>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s, s2)
+                : launch_stage_batch_v<S, PERT, 5, false>(args, ftiles, nf, gtiles, ng, stage, ncases, variant, s, s2);
 }
 
 #if defined(NS_KERNELS_F64)
 hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
-                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s)
+                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2)
 {
-    return launch_stage_batch_p<double, false>(args, ftiles, nf, gtiles, ng, stage, ncases, nk, visc, variant, s);
+    return launch_stage_batch_p<double, false>(args, ftiles, nf, gtiles, ng, stage, ncases, nk, visc, variant, s,
+                                                 s2);
 }
 #endif
 #if defined(NS_KERNELS_F32)
 hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
-                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s)
+                              int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2)
 {
-    return launch_stage_batch_p<float, true>(args, ftiles, nf, gtiles, ng, stage, ncases, nk, visc, variant, s);
+    return launch_stage_batch_p<float, true>(args, ftiles, nf, gtiles, ng, stage, ncases, nk, visc, variant, s,
+                                                 s2);
 }
 #endif
 
 #if defined(NS_KERNELS_F64)
-hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s)
+hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
+                        hipStream_t s2)
 {
-    return launch_stage_p<double, false>(A, nk, visc, rhs_only, variant, s);
+    return launch_stage_p<double, false>(A, nk, visc, rhs_only, variant, s, s2);
 }
 #endif
 
 #if defined(NS_KERNELS_F32)
-hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s)
+hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
+                        hipStream_t s2)
 {
-    return launch_stage_p<float, true>(A, nk, visc, rhs_only, variant, s);
+    return launch_stage_p<float, true>(A, nk, visc, rhs_only, variant, s, s2);
 }
 #endif
 
