@@ -151,6 +151,40 @@ int32_t ns_batch_get_state(ns_batch *b, int32_t c, double *U);
 int32_t ns_batch_autotune(ns_batch *b, int32_t *variant, double *ms);
 int32_t ns_batch_destroy(ns_batch *b);
 
+/* device-resident state ------------------------------------------------------
+   U points to device memory holding nk planes of imax*jmax FP64 values
+   (plane k, row j, column i at U[k*imax*jmax + j*imax + i]): the layout of a
+   contiguous (nk, jmax, imax) torch tensor.  Same meaning as ns_put_state /
+   ns_get_state, without the host round trip.  No reference counterpart. */
+int32_t ns_put_state_dev(ns_ctx *ctx, const double *U);
+int32_t ns_get_state_dev(ns_ctx *ctx, double *U);
+
+/* reverse mode ---------------------------------------------------------------
+   Discrete adjoint of the LDDRK6 step for the 2-D Euler equations (nk = 4,
+   NS_FLAG_INVISCID), both precision modes.  No reference counterpart: the
+   reference integrates forward only.
+   Objective: J = dt * sum_{n=1..nsteps} sum_arc (p_n - pb)^2, p_n the pressure
+   after step n at the observer points (1-based (i,j) pairs, each listed once).
+   ns_adj_forward runs nsteps from the current state (like ns_rk_steps),
+   keeping a checkpoint every `every` steps, and returns J.  ns_adj_backward
+   returns dJ/dU0 (host AoS like ns_get_state; _dev: device planes as
+   ns_put_state_dev) by recomputing each segment from its checkpoint; the
+   state is the final state again afterwards. */
+int32_t ns_adj_set_arc(ns_ctx *ctx, int32_t npts, const int32_t *ij);
+int32_t ns_adj_forward(ns_ctx *ctx, int32_t iter0, int32_t nsteps, int32_t every, double *J);
+int32_t ns_adj_backward(ns_ctx *ctx, double *dJdU0);
+int32_t ns_adj_backward_dev(ns_ctx *ctx, double *dJdU0);
+/* transposed operators (host AoS, FP64), the backward counterparts of
+   ns_rhs (lam_U = R'(U)^T lam_f at every point), ns_filter_x / ns_filter_y
+   and ns_ccpml (in place) */
+int32_t ns_vjp_rhs(ns_ctx *ctx, const double *U, const double *lam_f, double *lam_U);
+int32_t ns_vjp_filter_x(ns_ctx *ctx, double *lam);
+int32_t ns_vjp_filter_y(ns_ctx *ctx, double *lam);
+int32_t ns_vjp_ccpml(ns_ctx *ctx, double *lam);
+/* lam <- (dU_{n+1}/dU_n)^T lam for one step (iter) from state U (host AoS);
+   leaves the context's state at U */
+int32_t ns_vjp_step(ns_ctx *ctx, int32_t iter, const double *U, double *lam);
+
 /* tuning ---------------------------------------------------------------------- */
 /* Time the stage kernel's block-size variants (256, 512, 1024 threads on the
    same LDS tile; results are bitwise identical) on this context's workload and

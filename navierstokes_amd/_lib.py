@@ -6,6 +6,7 @@ raises instead of falling back to anything else.
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libnsamd.so"
@@ -87,13 +88,43 @@ SIGNATURES = {
     "ns_batch_get_state": (C.c_int32, [_V, C.c_int32, _DP]),
     "ns_batch_autotune": (C.c_int32, [_V, _IP, _DP]),
     "ns_batch_destroy": (C.c_int32, [_V]),
+    "ns_put_state_dev": (C.c_int32, [_V, _V]),
+    "ns_get_state_dev": (C.c_int32, [_V, _V]),
+    "ns_adj_set_arc": (C.c_int32, [_V, C.c_int32, _IP]),
+    "ns_adj_forward": (C.c_int32, [_V, C.c_int32, C.c_int32, C.c_int32, _DP]),
+    "ns_adj_backward": (C.c_int32, [_V, _DP]),
+    "ns_adj_backward_dev": (C.c_int32, [_V, _V]),
+    "ns_vjp_rhs": (C.c_int32, [_V, _DP, _DP, _DP]),
+    "ns_vjp_filter_x": (C.c_int32, [_V, _DP]),
+    "ns_vjp_filter_y": (C.c_int32, [_V, _DP]),
+    "ns_vjp_ccpml": (C.c_int32, [_V, _DP]),
+    "ns_vjp_step": (C.c_int32, [_V, C.c_int32, _DP, _DP]),
 }
 
 _lib = None
 
 
+def _torch_hip_runtime():
+    """torch's own libamdhip64.so, if torch is installed (found without importing it)."""
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    for loc in spec.submodule_search_locations:
+        p = Path(loc) / "lib" / "libamdhip64.so"
+        if p.exists():
+            return p
+    return None
+
+
 def load() -> C.CDLL:
-    """Load libnsamd.so (built by navierstokes_amd.build); raise if absent."""
+    """Load libnsamd.so (built by navierstokes_amd.build); raise if absent.
+
+    One process must hold one HIP runtime, or device pointers (torch tensors)
+    cannot cross between torch and the library.  torch loads its bundled
+    libamdhip64.so by file name; the library needs libamdhip64.so.7, the same
+    soname.  Preloading torch's copy first makes the library bind to it (soname
+    match) and a later `import torch` reuse it (same file), in either import
+    order.  Without torch the system ROCm runtime is used."""
     global _lib
     if _lib is not None:
         return _lib
@@ -101,6 +132,9 @@ def load() -> C.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -m navierstokes_amd.build` "
             "(the HIP library is the only compute path)")
+    rt = _torch_hip_runtime()
+    if rt is not None:
+        C.CDLL(str(rt), mode=C.RTLD_GLOBAL)
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

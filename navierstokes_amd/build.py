@@ -17,8 +17,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libnsamd.so"
 
-SOURCES = ["ns_kernels.hip", "ns_api.hip"]
-HEADERS = ["ns_device.h", "ns_kernels.h"]
+SOURCES = ["ns_kernels.hip", "ns_adjoint.hip", "ns_api.hip"]
+HEADERS = ["ns_device.h", "ns_kernels.h", "ns_adjoint.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -53,9 +53,9 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> Path:
     """Compile libnsamd.so for gfx950 (always recompiles).
 
-    ns_kernels.hip is compiled twice: the FP64 path with -ffp-contract=off
-    (reference operation order, no fused multiply-adds) and the FP32
-    perturbation form with contraction."""
+    ns_kernels.hip and ns_adjoint.hip are compiled twice: the FP64 path with
+    -ffp-contract=off (reference operation order, no fused multiply-adds) and
+    the FP32 perturbation form with contraction."""
     cc = hipcc()
     inc = ["-I", str(ROOT / "include"), "-I", str(CSRC)]
     base = [f for f in HIPCC_FLAGS if f != "-shared"]
@@ -65,6 +65,8 @@ def build(verbose: bool = False) -> Path:
     units = [
         ("ns_kernels.hip", "ns_kernels_f64.o", ["-DNS_KERNELS_F64", "-ffp-contract=off"]),
         ("ns_kernels.hip", "ns_kernels_f32.o", ["-DNS_KERNELS_F32"]),
+        ("ns_adjoint.hip", "ns_adjoint_f64.o", ["-DNS_KERNELS_F64", "-ffp-contract=off"]),
+        ("ns_adjoint.hip", "ns_adjoint_f32.o", ["-DNS_KERNELS_F32"]),
         ("ns_api.hip", "ns_api.o", []),
     ]
     for src, obj, extra in units:

@@ -301,6 +301,55 @@ def batch_sweep_cases(ncases: int = 256, seed: int = 0, sizes=(128, 192, 256, 38
     return out
 
 
+def adjoint_case(N: int = 1024, D: int = 40, M: float = 0.3, ngauss: int = 16, seed: int = 5,
+                 amp_sigma: float = 1e-3, hw_range=(3.0, 8.0), inner: float = 0.6) -> Case:
+    """2-D Euler on an N x N unit grid with uniform mean flow M; the initial
+    pressure perturbation is a seeded sum of `ngauss` Gaussians with
+    amplitudes ~ N(0, amp_sigma^2), centres uniform in the inner `inner`
+    fraction of the domain and half-widths ~ U(hw_range).  Density and
+    velocity start at the base state.  Returns the case with U0 set; the
+    perturbation itself is `adjoint_pressure(case_seed...)`."""
+    case = acoustic_pulse_case(N, D=D, amp_rel=0.0, u0=M)
+    dp = adjoint_pressure(N, ngauss, seed, amp_sigma, hw_range, inner)
+    case.U0 = case.Ub.copy()
+    case.U0[..., 3] += dp
+    case.name = f"euler-adjoint-{N}-M{M}"
+    return case
+
+
+def adjoint_pressure(N: int, ngauss: int = 16, seed: int = 5, amp_sigma: float = 1e-3,
+                     hw_range=(3.0, 8.0), inner: float = 0.6) -> np.ndarray:
+    """Seeded sum of Gaussians on an N x N unit grid centred on 0, shape (N, N)."""
+    rng = np.random.default_rng(seed)
+    half = (N - 1) / 2.0
+    x = np.arange(N) - half
+    X, Y = np.meshgrid(x, x)
+    dp = np.zeros((N, N))
+    for _ in range(ngauss):
+        a = rng.normal(0.0, amp_sigma)
+        xc, yc = rng.uniform(-inner * half, inner * half, size=2)
+        b = rng.uniform(*hw_range)
+        dp += a * np.exp(-math.log(2.0) * ((X - xc) ** 2 + (Y - yc) ** 2) / (b * b))
+    return dp
+
+
+def observer_arc(N: int, radius: float, theta0: float = 0.0, theta1: float = math.pi) -> np.ndarray:
+    """Grid points (1-based (i, j), each once) nearest to an arc of the given
+    radius about the centre of an N x N unit grid, sampled at one-cell
+    spacing from angle theta0 to theta1."""
+    half = (N - 1) / 2.0
+    nth = max(2, int(math.ceil(radius * (theta1 - theta0))) + 1)
+    pts = []
+    seen = set()
+    for t in np.linspace(theta0, theta1, nth):
+        i = int(round(half + radius * math.cos(t))) + 1
+        j = int(round(half + radius * math.sin(t))) + 1
+        if 1 <= i <= N and 1 <= j <= N and (i, j) not in seen:
+            seen.add((i, j))
+            pts.append((i, j))
+    return np.asarray(pts, dtype=np.int32)
+
+
 def random_case(imax: int, jmax: int, nk: int = 4, D: int = 6, seed: int = 0, Re: float = 500.0,
                 amp: float = 1e-2, dt: float = 0.2, sigmaf: float = 0.2, inviscid: bool = False,
                 mesh_random: bool = True) -> Case:

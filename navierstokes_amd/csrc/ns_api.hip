@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/ns_api.h"
+#include "ns_adjoint.h"
 #include "ns_kernels.h"
 
 namespace nsamd {
@@ -32,6 +33,19 @@ hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles,
                               int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
 hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
                               int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
+hipError_t launch_adj_stage(const AdjPointArgs<double> &P, const AdjGatherArgs<double> &G, hipStream_t s);
+hipError_t launch_adj_stage(const AdjPointArgs<float> &P, const AdjGatherArgs<float> &G, hipStream_t s);
+hipError_t launch_adj_filter(const AdjFilterArgs<double> &F, bool ydir, hipStream_t s);
+hipError_t launch_adj_filter(const AdjFilterArgs<float> &F, bool ydir, hipStream_t s);
+hipError_t launch_adj_edge(const AdjEdgeArgs<double> &E, hipStream_t s);
+hipError_t launch_adj_edge(const AdjEdgeArgs<float> &E, hipStream_t s);
+hipError_t launch_arc_reduce(const ArcArgs<double> &a, double *out, hipStream_t s);
+hipError_t launch_arc_reduce(const ArcArgs<float> &a, double *out, hipStream_t s);
+hipError_t launch_arc_seed(const ArcArgs<double> &a, double *lam, double weight, hipStream_t s);
+hipError_t launch_arc_seed(const ArcArgs<float> &a, float *lam, double weight, hipStream_t s);
+hipError_t launch_convert(const ConvArgs<double, double> &a, hipStream_t s);
+hipError_t launch_convert(const ConvArgs<double, float> &a, hipStream_t s);
+hipError_t launch_convert(const ConvArgs<float, double> &a, hipStream_t s);
 hipError_t launch_filter_batch(const FilterArgs<double> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
                                hipStream_t s);
 hipError_t launch_filter_batch(const FilterArgs<float> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
@@ -114,6 +128,21 @@ struct ns_ctx {
     hipEvent_t ev[2 * 24] = {};
     double tsum[TC_N] = {};
     long tcount[TC_N] = {};
+    // reverse mode: observer arc, checkpoints and adjoint work planes
+    struct Adj {
+        int2 *arc = nullptr;
+        int narc = 0;
+        double *jvals = nullptr;  // per-step objective values
+        size_t jbytes = 0;
+        void *ckpt = nullptr;     // state copies at steps 0, K, 2K, ... and the final state
+        size_t ckbytes = 0;
+        void *seg = nullptr;      // step states of one segment (K+1)
+        size_t segbytes = 0;
+        void *stg = nullptr;      // the 6 stage input states of one step
+        void *lam = nullptr, *lamW = nullptr, *gx = nullptr, *gy = nullptr;
+        int iter0 = 0, nsteps = 0, every = 0;
+        bool have_fwd = false;
+    } adj;
 };
 
 namespace {
@@ -304,51 +333,63 @@ double stage_amp(const ns_params &P, int irk, int iter)
     return P.amplitude * std::sin(P.wf * (((double)iter + RK_C[irk]) * P.dt));
 }
 
-// one LDDRK6 step on the device, rk.f90:52-73
+// one LDDRK6 stage on the device (rk.f90:52-73, stage irk of step iter):
+// state S0 -> stage kernel -> S1 -> x filter -> S2 -> y filter (+ rows
+// j=1,jmax) -> S0 -> columns i=1,imax
 template <typename S>
-int enqueue_step_t(ns_ctx *c, int iter, bool timed)
+int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed)
 {
     const ns_params &P = c->P;
     const bool visc = !(P.flags & NS_FLAG_INVISCID);
     const bool pert = c->pert;
     S *S0 = static_cast<S *>(c->S[0]), *S1 = static_cast<S *>(c->S[1]), *S2 = static_cast<S *>(c->S[2]);
-    for (int irk = 0; irk < 6; ++irk) {
-        auto tick = [&](int slot) {
-            if (timed) hipEventRecord(c->ev[slot], c->stream);
-        };
-        StageArgs<S> A = stage_args<S>(c, irk, S0, S1, stage_amp(P, irk, iter), pert);
-        tick(0);
-        // interior tiles on the context's stream, edge-ring tiles on the side
-        // stream (fork after the previous work, join before the filters)
-        HIPCHK(hipEventRecord(c->fork, c->stream));
-        HIPCHK(hipStreamWaitEvent(c->side, c->fork, 0));
-        HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream, c->side));
-        HIPCHK(hipEventRecord(c->join, c->side));
-        HIPCHK(hipStreamWaitEvent(c->stream, c->join, 0));
-        tick(1);
-        HIPCHK(launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, c->stream));
-        tick(2);
-        HIPCHK(launch_filter(filter_args<S>(c, false, S2, S0, true, pert), false, c->stream));
-        tick(3);
-        HIPCHK(launch_edge_i(edge_args<S>(c, S0, pert), c->stream));
-        tick(4);
-        if (timed) {
-            HIPCHK(hipEventSynchronize(c->ev[4]));
-            for (int t = 0; t < TC_N; ++t) {
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, c->ev[t], c->ev[t + 1]));
-                c->tsum[t] += ms;
-                c->tcount[t] += 1;
-            }
+    auto tick = [&](int slot) {
+        if (timed) hipEventRecord(c->ev[slot], c->stream);
+    };
+    StageArgs<S> A = stage_args<S>(c, irk, S0, S1, stage_amp(P, irk, iter), pert);
+    tick(0);
+    // interior tiles on the context's stream, edge-ring tiles on the side
+    // stream (fork after the previous work, join before the filters)
+    HIPCHK(hipEventRecord(c->fork, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side, c->fork, 0));
+    HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream, c->side));
+    HIPCHK(hipEventRecord(c->join, c->side));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->join, 0));
+    tick(1);
+    HIPCHK(launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, c->stream));
+    tick(2);
+    HIPCHK(launch_filter(filter_args<S>(c, false, S2, S0, true, pert), false, c->stream));
+    tick(3);
+    HIPCHK(launch_edge_i(edge_args<S>(c, S0, pert), c->stream));
+    tick(4);
+    if (timed) {
+        HIPCHK(hipEventSynchronize(c->ev[4]));
+        for (int t = 0; t < TC_N; ++t) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[t], c->ev[t + 1]));
+            c->tsum[t] += ms;
+            c->tcount[t] += 1;
         }
     }
     return NS_OK;
 }
 
-int enqueue_step(ns_ctx *c, int iter, bool timed)
+int enqueue_stage(ns_ctx *c, int irk, int iter, bool timed)
 {
-    return c->pert ? enqueue_step_t<float>(c, iter, timed) : enqueue_step_t<double>(c, iter, timed);
+    return c->pert ? enqueue_stage_t<float>(c, irk, iter, timed) : enqueue_stage_t<double>(c, irk, iter, timed);
 }
+
+// one LDDRK6 step on the device, rk.f90:52-73
+int enqueue_step_t(ns_ctx *c, int iter, bool timed)
+{
+    for (int irk = 0; irk < 6; ++irk) {
+        int rc = enqueue_stage(c, irk, iter, timed);
+        if (rc) return rc;
+    }
+    return NS_OK;
+}
+
+int enqueue_step(ns_ctx *c, int iter, bool timed) { return enqueue_step_t(c, iter, timed); }
 
 // Autotuner: time each block-size variant of the stage kernel on the live
 // workload and keep the fastest.  It runs stage 1 (which does not read W)
@@ -563,6 +604,9 @@ void release(ns_ctx *c)
     if (c->fork) hipEventDestroy(c->fork);
     if (c->join) hipEventDestroy(c->join);
     if (c->side) hipStreamDestroy(c->side);
+    for (void *p : {(void *)c->adj.arc, (void *)c->adj.jvals, c->adj.ckpt, c->adj.seg, c->adj.stg, c->adj.lam,
+                    c->adj.lamW, c->adj.gx, c->adj.gy})
+        if (p) hipFree(p);
 }
 
 }  // namespace
@@ -1342,6 +1386,468 @@ int32_t ns_batch_destroy(ns_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_release(b);
     delete b;
+    return NS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reverse mode: discrete adjoint of the LDDRK6 step (2-D Euler, nk = 4).
+// No reference counterpart: the reference has no adjoint.  The forward keeps
+// checkpoints every K steps; the backward recomputes each segment's step
+// states from its checkpoint, each step's six stage inputs from its step
+// state, and runs the backward kernels stage by stage in reverse.
+// ---------------------------------------------------------------------------
+namespace {
+
+size_t state_bytes(const ns_ctx *c) { return (size_t)c->P.nk * c->plane * c->esz; }
+
+template <typename S>
+S *state_slot(void *base, const ns_ctx *c, int n)
+{
+    return static_cast<S *>(base) + (size_t)n * c->P.nk * c->plane;
+}
+
+int grow(void **p, size_t &have, size_t need)
+{
+    if (need <= have && *p) return NS_OK;
+    if (*p) HIPCHK(hipFree(*p));
+    *p = nullptr;
+    have = 0;
+    hipError_t e = hipMalloc(p, need);
+    if (e != hipSuccess) return fail(NS_ERR_NOMEM, "hipMalloc(%zu): %s", need, hipGetErrorString(e));
+    have = need;
+    return NS_OK;
+}
+
+int adj_check(ns_ctx *c)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!c->have_state) return fail(NS_ERR_STATE, "no state has been set");
+    if (c->P.nk != 4 || !(c->P.flags & NS_FLAG_INVISCID))
+        return fail(NS_ERR_PARAM, "the adjoint covers the 2-D Euler equations (nk = 4 with NS_FLAG_INVISCID)");
+    return NS_OK;
+}
+
+int adj_work(ns_ctx *c)
+{
+    const size_t b = state_bytes(c);
+    size_t have;
+    int rc;
+    for (void **pp : {&c->adj.lam, &c->adj.lamW, &c->adj.gx, &c->adj.gy}) {
+        have = *pp ? b : 0;
+        if ((rc = grow(pp, have, b))) return rc;
+    }
+    have = c->adj.stg ? 6 * b : 0;
+    return grow(&c->adj.stg, have, 6 * b);
+}
+
+hipError_t copy_state(ns_ctx *c, void *dst, const void *src)
+{
+    return hipMemcpyAsync(dst, src, state_bytes(c), hipMemcpyDeviceToDevice, c->stream);
+}
+
+template <typename S>
+ArcArgs<S> arc_args(ns_ctx *c, const S *state)
+{
+    ArcArgs<S> a{};
+    a.pts = c->adj.arc;
+    a.npts = c->adj.narc;
+    a.U = state + 3 * c->plane;
+    a.base = c->pert ? nullptr : plane_ptr(c->base, c, 3);
+    a.pitch = c->pitch;
+    return a;
+}
+
+template <typename S>
+void set_planes(S **dst, S *buf, const ns_ctx *c)
+{
+    for (int k = 0; k < NKMAX; ++k) dst[k] = k < c->P.nk ? buf + (size_t)k * c->plane : nullptr;
+}
+
+// ccpml^T, Fy^T, Fx^T on lam (in place)
+template <typename S>
+int adj_linear_back(ns_ctx *c, S *lam, bool edge, bool fy, bool fx)
+{
+    const ns_params &P = c->P;
+    if (edge) {
+        AdjEdgeArgs<S> E{};
+        set_planes(E.lam, lam, c);
+        E.imax = P.imax;
+        E.jmax = P.jmax;
+        E.pitch = c->pitch;
+        E.nk = P.nk;
+        HIPCHK(launch_adj_edge(E, c->stream));
+    }
+    AdjFilterArgs<S> F{};
+    set_planes(F.lam, lam, c);
+    F.pitch = c->pitch;
+    F.nk = P.nk;
+    F.sigma = P.sigmaf;
+    if (fy) {
+        F.n = P.jmax;
+        F.m = P.imax;
+        HIPCHK(launch_adj_filter(F, true, c->stream));
+    }
+    if (fx) {
+        F.n = P.imax;
+        F.m = P.jmax;
+        HIPCHK(launch_adj_filter(F, false, c->stream));
+    }
+    return NS_OK;
+}
+
+// (lam, lamW) <- backward of stage irk with input state Us; gx, gy: work planes
+template <typename S>
+int adj_stage_back(ns_ctx *c, int irk, S *Us, S *lam, S *lamW, S *gx, S *gy, bool pert, bool rhs_only)
+{
+    const ns_params &P = c->P;
+    if (!rhs_only) {
+        int rc = adj_linear_back<S>(c, lam, true, true, true);
+        if (rc) return rc;
+    }
+    AdjPointArgs<S> A{};
+    A.st = stage_args<S>(c, irk, Us, Us, 0.0, pert);
+    for (int k = 0; k < NKMAX; ++k) {
+        const bool on = k < P.nk;
+        A.lam[k] = on ? lam + (size_t)k * c->plane : nullptr;
+        A.lamW[k] = on && lamW ? lamW + (size_t)k * c->plane : nullptr;
+        A.lamWo[k] = on && lamW ? lamW + (size_t)k * c->plane : nullptr;
+        A.acc[k] = on ? lam + (size_t)k * c->plane : nullptr;
+        A.gx[k] = on ? gx + (size_t)k * c->plane : nullptr;
+        A.gy[k] = on ? gy + (size_t)k * c->plane : nullptr;
+    }
+    A.pert = pert;
+    A.rhs_only = rhs_only;
+    AdjGatherArgs<S> G{};
+    for (int k = 0; k < NKMAX; ++k) {
+        G.acc[k] = A.acc[k];
+        G.gx[k] = A.gx[k];
+        G.gy[k] = A.gy[k];
+        G.out[k] = A.acc[k];
+    }
+    G.imax = P.imax;
+    G.jmax = P.jmax;
+    G.pitch = c->pitch;
+    G.nk = P.nk;
+    G.inv_dm = 1.0 / P.dm;
+    G.inv_dn = 1.0 / P.dn;
+    HIPCHK(launch_adj_stage(A, G, c->stream));
+    return NS_OK;
+}
+
+// lam <- (d U_{n+1} / d U_n)^T lam for the step from the state at U0
+template <typename S>
+int adj_step_back(ns_ctx *c, int iter, const S *U0, S *lam)
+{
+    S *stg = static_cast<S *>(c->adj.stg);
+    HIPCHK(copy_state(c, c->S[0], U0));
+    for (int irk = 0; irk < 6; ++irk) {
+        HIPCHK(copy_state(c, state_slot<S>(stg, c, irk), c->S[0]));
+        if (irk < 5) {
+            int rc = enqueue_stage(c, irk, iter, false);
+            if (rc) return rc;
+        }
+    }
+    // the register W is dead after stage 6 (stage 1 of the next step does not read it)
+    HIPCHK(hipMemsetAsync(c->adj.lamW, 0, state_bytes(c), c->stream));
+    for (int irk = 5; irk >= 0; --irk) {
+        int rc = adj_stage_back<S>(c, irk, state_slot<S>(stg, c, irk), lam, static_cast<S *>(c->adj.lamW),
+                                   static_cast<S *>(c->adj.gx), static_cast<S *>(c->adj.gy), c->pert, false);
+        if (rc) return rc;
+    }
+    return NS_OK;
+}
+
+template <typename S>
+int adj_forward_t(ns_ctx *c, int iter0, int nsteps, int every, double *J)
+{
+    auto &a = c->adj;
+    const int nseg = (nsteps + every - 1) / every;
+    const size_t b = state_bytes(c);
+    int rc;
+    if ((rc = adj_work(c)) || (rc = grow(&a.ckpt, a.ckbytes, (size_t)(nseg + 1) * b)) ||
+        (rc = grow(&a.seg, a.segbytes, (size_t)(every + 1) * b)) ||
+        (rc = grow((void **)&a.jvals, a.jbytes, sizeof(double) * (size_t)std::max(nsteps, 1))))
+        return rc;
+    S *S0 = static_cast<S *>(c->S[0]);
+    for (int n = 0; n < nsteps; ++n) {
+        if (n % every == 0) HIPCHK(copy_state(c, state_slot<S>(a.ckpt, c, n / every), S0));
+        if ((rc = enqueue_step(c, iter0 + n, false))) return rc;
+        HIPCHK(launch_arc_reduce(arc_args<S>(c, S0), a.jvals + n, c->stream));
+    }
+    HIPCHK(copy_state(c, state_slot<S>(a.ckpt, c, nseg), S0));
+    std::vector<double> jv((size_t)nsteps);
+    if (nsteps > 0)
+        HIPCHK(hipMemcpyAsync(jv.data(), a.jvals, sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double sum = 0.0;
+    for (double v : jv) sum += v;
+    if (J) *J = c->P.dt * sum;
+    a.iter0 = iter0;
+    a.nsteps = nsteps;
+    a.every = every;
+    a.have_fwd = true;
+    return NS_OK;
+}
+
+// dJ/dU0 into adj.lam; the state is restored to the final state afterwards
+template <typename S>
+int adj_backward_t(ns_ctx *c)
+{
+    auto &a = c->adj;
+    const int N = a.nsteps, K = a.every;
+    const int nseg = (N + K - 1) / K;
+    S *lam = static_cast<S *>(a.lam);
+    S *S0 = static_cast<S *>(c->S[0]);
+    HIPCHK(hipMemsetAsync(lam, 0, state_bytes(c), c->stream));
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * K, s1 = std::min(N, s0 + K);
+        HIPCHK(copy_state(c, S0, state_slot<S>(a.ckpt, c, sg)));
+        HIPCHK(copy_state(c, state_slot<S>(a.seg, c, 0), S0));
+        for (int n = s0; n < s1; ++n) {
+            int rc = enqueue_step(c, a.iter0 + n, false);
+            if (rc) return rc;
+            HIPCHK(copy_state(c, state_slot<S>(a.seg, c, n - s0 + 1), S0));
+        }
+        for (int n = s1 - 1; n >= s0; --n) {
+            // J = dt sum_m phi(U_m), m = 1..N: seed dphi/dU at U_{n+1}, then step n back
+            HIPCHK(launch_arc_seed(arc_args<S>(c, state_slot<S>(a.seg, c, n - s0 + 1)), lam + 3 * c->plane, c->P.dt,
+                                   c->stream));
+            int rc = adj_step_back<S>(c, a.iter0 + n, state_slot<S>(a.seg, c, n - s0), lam);
+            if (rc) return rc;
+        }
+    }
+    HIPCHK(copy_state(c, S0, state_slot<S>(a.ckpt, c, nseg)));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+// device planes (nk x imax*jmax, FP64) <-> a state-precision pitched buffer
+template <typename S>
+int dev_to_state(ns_ctx *c, const double *src, S *dst, bool sub_base)
+{
+    ConvArgs<double, S> a{};
+    a.in = src;
+    a.out = dst;
+    a.ipitch = c->P.imax;
+    a.opitch = c->pitch;
+    a.imax = c->P.imax;
+    a.jmax = c->P.jmax;
+    a.nk = c->P.nk;
+    a.iplane = (size_t)c->P.imax * c->P.jmax;
+    a.oplane = c->plane;
+    a.sgn = 1.0;
+    if (sub_base && c->pert) {
+        const PertRows pr = pert_rows(c);
+        for (int k = 0; k < c->P.nk; ++k) a.sub[k] = pr.ub[k];
+    }
+    HIPCHK(launch_convert(a, c->stream));
+    return NS_OK;
+}
+
+template <typename S>
+int state_to_dev(ns_ctx *c, const S *src, double *dst, bool add_base)
+{
+    ConvArgs<S, double> a{};
+    a.in = src;
+    a.out = dst;
+    a.ipitch = c->pitch;
+    a.opitch = c->P.imax;
+    a.imax = c->P.imax;
+    a.jmax = c->P.jmax;
+    a.nk = c->P.nk;
+    a.iplane = c->plane;
+    a.oplane = (size_t)c->P.imax * c->P.jmax;
+    a.sgn = -1.0;
+    if (add_base && c->pert) {
+        const PertRows pr = pert_rows(c);
+        for (int k = 0; k < c->P.nk; ++k) a.sub[k] = pr.ub[k];
+    }
+    HIPCHK(launch_convert(a, c->stream));
+    return NS_OK;
+}
+
+// host AoS (FP64) <-> state-precision pitched buffer (via the FP64 work planes)
+template <typename S>
+int host_to_buf(ns_ctx *c, const double *host, S *dst)
+{
+    if (sizeof(S) == sizeof(double)) return upload_aos(c, reinterpret_cast<double *>(dst), host, c->P.nk);
+    double *tmp = c->work;
+    int rc = upload_aos(c, tmp, host, c->P.nk);
+    if (rc) return rc;
+    ConvArgs<double, S> a{};
+    a.in = tmp;
+    a.out = dst;
+    a.ipitch = a.opitch = c->pitch;
+    a.imax = c->P.imax;
+    a.jmax = c->P.jmax;
+    a.nk = c->P.nk;
+    a.iplane = a.oplane = c->plane;
+    HIPCHK(launch_convert(a, c->stream));
+    return NS_OK;
+}
+
+template <typename S>
+int buf_to_host(ns_ctx *c, const S *src, double *host)
+{
+    if (sizeof(S) == sizeof(double)) return download_aos(c, reinterpret_cast<const double *>(src), host, c->P.nk);
+    double *tmp = c->work;
+    ConvArgs<S, double> a{};
+    a.in = src;
+    a.out = tmp;
+    a.ipitch = a.opitch = c->pitch;
+    a.imax = c->P.imax;
+    a.jmax = c->P.jmax;
+    a.nk = c->P.nk;
+    a.iplane = a.oplane = c->plane;
+    HIPCHK(launch_convert(a, c->stream));
+    return download_aos(c, tmp, host, c->P.nk);
+}
+
+template <typename S>
+int vjp_step_t(ns_ctx *c, int iter, const double *U, double *lam)
+{
+    int rc;
+    if ((rc = adj_work(c)) || (rc = grow(&c->adj.seg, c->adj.segbytes, state_bytes(c)))) return rc;
+    if ((rc = upload_state(c, U))) return rc;
+    HIPCHK(copy_state(c, c->adj.seg, c->S[0]));
+    S *l = static_cast<S *>(c->adj.lam);
+    if ((rc = host_to_buf<S>(c, lam, l))) return rc;
+    if ((rc = adj_step_back<S>(c, iter, static_cast<const S *>(c->adj.seg), l))) return rc;
+    HIPCHK(copy_state(c, c->S[0], c->adj.seg));
+    return buf_to_host<S>(c, l, lam);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ns_adj_set_arc(ns_ctx *c, int32_t npts, const int32_t *ij)
+{
+    if (!c || npts < 0 || (npts > 0 && !ij)) return fail(NS_ERR_PARAM, "bad argument");
+    std::vector<int2> pts((size_t)npts);
+    std::vector<char> seen((size_t)c->P.imax * c->P.jmax, 0);
+    for (int t = 0; t < npts; ++t) {
+        const int i = ij[2 * t], j = ij[2 * t + 1];
+        if (i < 1 || i > c->P.imax || j < 1 || j > c->P.jmax)
+            return fail(NS_ERR_PARAM, "observer point %d at (%d,%d) lies outside the grid", t, i, j);
+        char &s = seen[(size_t)(j - 1) * c->P.imax + (i - 1)];
+        if (s) return fail(NS_ERR_PARAM, "observer point %d at (%d,%d) is listed twice", t, i, j);
+        s = 1;
+        pts[t] = make_int2(i - 1, j - 1);
+    }
+    if (c->adj.arc) HIPCHK(hipFree(c->adj.arc));
+    c->adj.arc = nullptr;
+    c->adj.narc = 0;
+    if (npts > 0) {
+        HIPCHK(hipMalloc(&c->adj.arc, sizeof(int2) * npts));
+        HIPCHK(hipMemcpy(c->adj.arc, pts.data(), sizeof(int2) * npts, hipMemcpyHostToDevice));
+    }
+    c->adj.narc = npts;
+    c->adj.have_fwd = false;
+    return NS_OK;
+}
+
+int32_t ns_adj_forward(ns_ctx *c, int32_t iter0, int32_t nsteps, int32_t every, double *J)
+{
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (nsteps < 1 || every < 1) return fail(NS_ERR_PARAM, "nsteps and the checkpoint interval must be >= 1");
+    if (c->stage_variant < 0 && (rc = autotune_stage(c))) return rc;
+    return c->pert ? adj_forward_t<float>(c, iter0, nsteps, every, J) : adj_forward_t<double>(c, iter0, nsteps, every, J);
+}
+
+static int adj_backward_common(ns_ctx *c)
+{
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!c->adj.have_fwd) return fail(NS_ERR_STATE, "ns_adj_forward has not been run");
+    return c->pert ? adj_backward_t<float>(c) : adj_backward_t<double>(c);
+}
+
+int32_t ns_adj_backward(ns_ctx *c, double *dJdU0)
+{
+    if (!c || !dJdU0) return fail(NS_ERR_PARAM, "null argument");
+    int rc = adj_backward_common(c);
+    if (rc) return rc;
+    return c->pert ? buf_to_host<float>(c, static_cast<const float *>(c->adj.lam), dJdU0)
+                   : buf_to_host<double>(c, static_cast<const double *>(c->adj.lam), dJdU0);
+}
+
+int32_t ns_adj_backward_dev(ns_ctx *c, double *dJdU0)
+{
+    if (!c || !dJdU0) return fail(NS_ERR_PARAM, "null argument");
+    int rc = adj_backward_common(c);
+    if (rc) return rc;
+    rc = c->pert ? state_to_dev<float>(c, static_cast<const float *>(c->adj.lam), dJdU0, false)
+                 : state_to_dev<double>(c, static_cast<const double *>(c->adj.lam), dJdU0, false);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int32_t ns_put_state_dev(ns_ctx *c, const double *U)
+{
+    if (!c || !U) return fail(NS_ERR_PARAM, "null argument");
+    if (c->pert && !c->have_base) return fail(NS_ERR_STATE, "the FP32 perturbation form needs ns_set_base first");
+    int rc = c->pert ? dev_to_state<float>(c, U, static_cast<float *>(c->S[0]), true)
+                     : dev_to_state<double>(c, U, static_cast<double *>(c->S[0]), true);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return NS_OK;
+}
+
+int32_t ns_get_state_dev(ns_ctx *c, double *U)
+{
+    if (!c || !U) return fail(NS_ERR_PARAM, "null argument");
+    int rc = c->pert ? state_to_dev<float>(c, static_cast<const float *>(c->S[0]), U, true)
+                     : state_to_dev<double>(c, static_cast<const double *>(c->S[0]), U, true);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int32_t ns_vjp_rhs(ns_ctx *c, const double *U, const double *lam_f, double *lam_U)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!U || !lam_f || !lam_U) return fail(NS_ERR_PARAM, "null argument");
+    if (c->P.nk != 4 || !(c->P.flags & NS_FLAG_INVISCID))
+        return fail(NS_ERR_PARAM, "the adjoint covers the 2-D Euler equations (nk = 4 with NS_FLAG_INVISCID)");
+    // FP64 operator mirror: U, lam and the flux adjoints in the FP64 work planes
+    double *dU = c->work, *dl = c->work + 4 * c->plane, *gx = c->work + 8 * c->plane, *gy = c->work + 12 * c->plane;
+    if ((rc = upload_aos(c, dU, U, 4)) || (rc = upload_aos(c, dl, lam_f, 4))) return rc;
+    if ((rc = adj_stage_back<double>(c, 0, dU, dl, nullptr, gx, gy, false, true))) return rc;
+    return download_aos(c, dl, lam_U, 4);
+}
+
+static int vjp_linear(ns_ctx *c, double *lam, bool edge, bool fy, bool fx)
+{
+    if (!c || !lam) return fail(NS_ERR_PARAM, "null argument");
+    if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
+    double *d = c->work;
+    int rc;
+    if ((rc = upload_aos(c, d, lam, c->P.nk))) return rc;
+    if ((rc = adj_linear_back<double>(c, d, edge, fy, fx))) return rc;
+    return download_aos(c, d, lam, c->P.nk);
+}
+
+int32_t ns_vjp_filter_x(ns_ctx *c, double *lam) { return vjp_linear(c, lam, false, false, true); }
+int32_t ns_vjp_filter_y(ns_ctx *c, double *lam) { return vjp_linear(c, lam, false, true, false); }
+int32_t ns_vjp_ccpml(ns_ctx *c, double *lam) { return vjp_linear(c, lam, true, false, false); }
+
+int32_t ns_vjp_step(ns_ctx *c, int32_t iter, const double *U, double *lam)
+{
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!U || !lam) return fail(NS_ERR_PARAM, "null argument");
+    if (c->stage_variant < 0 && (rc = autotune_stage(c))) return rc;
+    rc = c->pert ? vjp_step_t<float>(c, iter, U, lam) : vjp_step_t<double>(c, iter, U, lam);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return NS_OK;
 }
 

@@ -14,6 +14,9 @@ reference) and exposes the reference routines under their own names:
     ccpml(U)                       cc.f90:6
     acousticfont(ct, iter)         rk.f90:171
 
+Reverse mode (no reference counterpart; 2-D Euler): adj_set_arc /
+adj_forward / adj_backward and the transposed operators vjp_*.
+
 Host arrays follow the reference layout U(1:nk,1:imax,1:jmax), i.e. numpy
 shape (jmax, imax, nk), C-contiguous, float64.  Errors raise NSError.
 """
@@ -189,6 +192,63 @@ class Solver:
         s = np.empty((self.case.jmax, self.case.imax))
         _lib.check(self.lib.ns_acousticfont(self.ctx, float(ct), int(iter), _lib.dptr(s)))
         return s
+
+    # device-resident state (torch tensors of shape (nk, jmax, imax), float64) --
+    def put_state_dev(self, U):
+        self._check_dev(U)
+        _lib.check(self.lib.ns_put_state_dev(self.ctx, C.c_void_p(U.data_ptr())))
+
+    def get_state_dev(self, out):
+        self._check_dev(out)
+        _lib.check(self.lib.ns_get_state_dev(self.ctx, C.c_void_p(out.data_ptr())))
+
+    def _check_dev(self, t):
+        c = self.case
+        if (tuple(t.shape) != (c.nk, c.jmax, c.imax) or str(t.dtype) != "torch.float64" or not t.is_cuda
+                or not t.is_contiguous()):
+            raise ValueError(f"expected a contiguous float64 device tensor of shape {(c.nk, c.jmax, c.imax)}")
+
+    # reverse mode -------------------------------------------------------------
+    def adj_set_arc(self, points):
+        """Observer points, 1-based (i, j), each once."""
+        ij = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 2))
+        _lib.check(self.lib.ns_adj_set_arc(self.ctx, ij.shape[0], ij.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def adj_forward(self, iter0: int, nsteps: int, every: int) -> float:
+        J = C.c_double()
+        _lib.check(self.lib.ns_adj_forward(self.ctx, int(iter0), int(nsteps), int(every), C.byref(J)))
+        return J.value
+
+    def adj_backward(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        _lib.check(self.lib.ns_adj_backward(self.ctx, _lib.dptr(out)))
+        return out
+
+    def adj_backward_dev(self, out):
+        self._check_dev(out)
+        _lib.check(self.lib.ns_adj_backward_dev(self.ctx, C.c_void_p(out.data_ptr())))
+
+    def vjp_rhs(self, U, lam_f) -> np.ndarray:
+        U = _f64(U, self.shape)
+        lam_f = _f64(lam_f, self.shape)
+        out = np.empty(self.shape)
+        _lib.check(self.lib.ns_vjp_rhs(self.ctx, _lib.dptr(U), _lib.dptr(lam_f), _lib.dptr(out)))
+        return out
+
+    def vjp_filter_x(self, lam):
+        return self._inplace(self.lib.ns_vjp_filter_x, lam)
+
+    def vjp_filter_y(self, lam):
+        return self._inplace(self.lib.ns_vjp_filter_y, lam)
+
+    def vjp_ccpml(self, lam):
+        return self._inplace(self.lib.ns_vjp_ccpml, lam)
+
+    def vjp_step(self, iter: int, U, lam) -> np.ndarray:
+        U = _f64(U, self.shape)
+        lam = _f64(lam, self.shape).copy()
+        _lib.check(self.lib.ns_vjp_step(self.ctx, int(iter), _lib.dptr(U), _lib.dptr(lam)))
+        return lam
 
     # tuning / measurement -----------------------------------------------------
     def autotune(self):
