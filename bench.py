@@ -13,6 +13,12 @@ together by the batched path (one launch per kernel class, graph per step),
 FP32-perturbation form by default; reports the aggregate rate and the p'
 error against the FP64 batch after the same steps.
 
+`--workload adjoint`: dJ/d(initial p) of the 2-D Euler equations on 1024^2
+(M=0.3, 16 seeded Gaussians), J = time-integrated p'^2 on a radius-400
+observer arc, 500 steps with checkpoints every 50, FP64 and the FP32
+perturbation form; reports forward and backward times and checks the FP64
+gradient against central differences on 32 seeded directions.
+
 Prints one JSON line: Mpts*steps/s over the timed steps, the roofline of the
 dominant kernel (stage kernel, HBM bound) from HIP events on the library's
 stream, the whole-step roofline against the 704 B/pt/step compulsory
@@ -159,12 +165,96 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args):
+    from navierstokes_amd import config
+    from navierstokes_amd.solver import Solver
+
+    N = args.size or 1024
+    nsteps = args.steps if args.steps != 50 else 500
+    every = args.every
+    case = config.adjoint_case(N, D=40, M=0.3, ngauss=16, seed=5)
+    arc = config.observer_arc(N, 0.390625 * N)  # radius 400 at 1024^2
+    dp = case.U0[..., 3] - case.Ub[..., 3]
+    desc = (f"euler-adjoint {N}x{N} M=0.3 16 gaussians, observer arc r={0.390625 * N:.0f} "
+            f"({len(arc)} pts), {nsteps} steps, checkpoint every {every}")
+    print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
+
+    def J_of(s, x, chk):
+        U0 = case.Ub.copy()
+        U0[..., 3] += x
+        s.put_state(U0)
+        return s.adj_forward(0, nsteps, chk)
+
+    res = {}
+    grads = {}
+    for prec, name in ((0, "fp64"), (1, "fp32pert")):
+        s = Solver(case, precision=prec)
+        s.adj_set_arc(arc)
+        s.autotune()
+        J_of(s, dp, every)  # warm-up (kernels, buffers)
+        s.synchronize()
+        t0 = time.perf_counter()
+        J = J_of(s, dp, every)
+        t1 = time.perf_counter()
+        g = s.adj_backward()
+        t2 = time.perf_counter()
+        grads[name] = g
+        res[name] = {"J": J, "forward_s": t1 - t0, "backward_s": t2 - t1,
+                     "forward_ms_per_step": (t1 - t0) / nsteps * 1e3,
+                     "backward_ms_per_step": (t2 - t1) / nsteps * 1e3}
+        print(f"[bench] {name}: J={J:.6e} forward {t1 - t0:.2f} s backward {t2 - t1:.2f} s",
+              file=sys.stderr, flush=True)
+        if prec == 0 and args.fd_directions > 0:
+            rng = np.random.default_rng(7)
+            g3 = g[..., 3]
+            errs = []
+            for t in range(args.fd_directions):
+                d = rng.standard_normal(dp.shape)
+                d *= np.linalg.norm(dp) / np.linalg.norm(d)
+                eps = 1e-3
+                fd = (J_of(s, dp + eps * d, nsteps) - J_of(s, dp - eps * d, nsteps)) / (2 * eps)
+                ad = float(np.sum(g3 * d))
+                errs.append(abs(fd - ad) / abs(fd))
+            res[name]["fd_check"] = {"directions": args.fd_directions, "eps_rel": 1e-3,
+                                     "max_rel_err": max(errs), "median_rel_err": float(np.median(errs))}
+            print(f"[bench] fd check: max rel err {max(errs):.2e}", file=sys.stderr, flush=True)
+        s.close()
+    g64, g32 = grads["fp64"], grads["fp32pert"]
+    res["fp32pert"]["grad_p_rel_l2_vs_fp64"] = float(np.linalg.norm(g32[..., 3] - g64[..., 3])
+                                                      / np.linalg.norm(g64[..., 3]))
+    prec = args.precision or "fp64"
+    r = res[prec]
+    t = r["forward_s"] + r["backward_s"]
+    npts = case.npoints
+    out = {
+        "metric": METRIC,
+        "value": round(npts * nsteps / t / 1e6, 3),
+        "unit": "Mpts·steps/s",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": 1,
+        "ms_per_step": round(t / nsteps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32pert" else "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "grid": [N, N], "nk": 4, "precision": prec, "parallelism": "single-gpu",
+                   "step": "forward + reverse (gradient) LDDRK6 step"},
+        "roofline": None,
+        "adjoint": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["ns", "pulse", "batch"], default="ns")
+    ap.add_argument("--workload", choices=["ns", "pulse", "batch", "adjoint"], default="ns")
+    ap.add_argument("--fd-directions", type=int, default=32)
+    ap.add_argument("--every", type=int, default=50, help="adjoint checkpoint interval")
     ap.add_argument("--ncases", type=int, default=256)
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -185,6 +275,9 @@ def main():
 
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
     args.precision = args.precision or "fp64"
 

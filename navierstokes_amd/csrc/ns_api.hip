@@ -1568,7 +1568,6 @@ int adj_forward_t(ns_ctx *c, int iter0, int nsteps, int every, double *J)
     const size_t b = state_bytes(c);
     int rc;
     if ((rc = adj_work(c)) || (rc = grow(&a.ckpt, a.ckbytes, (size_t)(nseg + 1) * b)) ||
-        (rc = grow(&a.seg, a.segbytes, (size_t)(every + 1) * b)) ||
         (rc = grow((void **)&a.jvals, a.jbytes, sizeof(double) * (size_t)std::max(nsteps, 1))))
         return rc;
     S *S0 = static_cast<S *>(c->S[0]);
@@ -1599,6 +1598,8 @@ int adj_backward_t(ns_ctx *c)
     auto &a = c->adj;
     const int N = a.nsteps, K = a.every;
     const int nseg = (N + K - 1) / K;
+    int rc = grow(&a.seg, a.segbytes, (size_t)(K + 1) * state_bytes(c));
+    if (rc) return rc;
     S *lam = static_cast<S *>(a.lam);
     S *S0 = static_cast<S *>(c->S[0]);
     HIPCHK(hipMemsetAsync(lam, 0, state_bytes(c), c->stream));
@@ -1607,16 +1608,14 @@ int adj_backward_t(ns_ctx *c)
         HIPCHK(copy_state(c, S0, state_slot<S>(a.ckpt, c, sg)));
         HIPCHK(copy_state(c, state_slot<S>(a.seg, c, 0), S0));
         for (int n = s0; n < s1; ++n) {
-            int rc = enqueue_step(c, a.iter0 + n, false);
-            if (rc) return rc;
+            if ((rc = enqueue_step(c, a.iter0 + n, false))) return rc;
             HIPCHK(copy_state(c, state_slot<S>(a.seg, c, n - s0 + 1), S0));
         }
         for (int n = s1 - 1; n >= s0; --n) {
             // J = dt sum_m phi(U_m), m = 1..N: seed dphi/dU at U_{n+1}, then step n back
             HIPCHK(launch_arc_seed(arc_args<S>(c, state_slot<S>(a.seg, c, n - s0 + 1)), lam + 3 * c->plane, c->P.dt,
                                    c->stream));
-            int rc = adj_step_back<S>(c, a.iter0 + n, state_slot<S>(a.seg, c, n - s0), lam);
-            if (rc) return rc;
+            if ((rc = adj_step_back<S>(c, a.iter0 + n, state_slot<S>(a.seg, c, n - s0), lam))) return rc;
         }
     }
     HIPCHK(copy_state(c, S0, state_slot<S>(a.ckpt, c, nseg)));
