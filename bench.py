@@ -289,6 +289,7 @@ def main():
         desc = desc.replace("fp64", "fp32-perturbation")
     print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
     s = Solver(case, precision=prec)
+    variant, tune_ms = s.autotune()
     s.run(0, args.warmup)
     s.synchronize()
     print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
@@ -327,7 +328,8 @@ def main():
         "dtype": "f32" if prec else "f64",
         "data": "synthetic",
         "config": {"workload": desc, "grid": [case.imax, case.jmax], "nk": nk, "precision": args.precision,
-                   "viscous": not case.inviscid, "parallelism": "single-gpu"},
+                   "viscous": not case.inviscid, "parallelism": "single-gpu", "stage_variant": variant,
+                   "stage_variant_ms": [round(x, 4) for x in tune_ms]},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "stage_kernel (RHS+LDDRK update+buffer)",

@@ -92,37 +92,70 @@ __host__ __device__ inline void fast_range(int fast_ok, int D, int n, int T, int
     if (!fast_ok || n - Dm - 3 - T < 0 || b1 < b0) b0 = b1 = 0;
 }
 
-// The tiles of an nbx x nby tile grid outside the fast rectangle
-// [fx0,fx1) x [fy0,fy1), enumerated as bottom band, top band, then the left
-// and right strips row by row.
-struct RingMap {
-    int nbx, nby, fx0, fx1, fy0, fy1;
-    __host__ __device__ int count() const
+// Up to four rectangles [x0,x1) x [y0,y1), each cut into 64 x ty tiles from
+// its own origin: the grid around an inner rectangle (bottom band, top band,
+// left and right strips), or the whole grid when the inner one is empty.
+struct RectSet {
+    int x0[4], x1[4], y0[4], y1[4];
+    int tx[4];      // tiles across each rectangle
+    int start[5];   // first tile of each rectangle (start[4] = total)
+    int ty;
+    __host__ __device__ int count() const { return start[4]; }
+    __host__ __device__ void tile(int t, int &i0, int &i1, int &j0, int &j1) const
     {
-        return (fy0 + nby - fy1) * nbx + (fy1 - fy0) * (fx0 + nbx - fx1);
+        int r = 0;
+        while (r < 3 && t >= start[r + 1]) ++r;
+        t -= start[r];
+        const int by = t / tx[r], bx = t - by * tx[r];
+        i0 = x0[r] + bx * 64;
+        i1 = i0 + 64 < x1[r] ? i0 + 64 : x1[r];
+        j0 = y0[r] + by * ty;
+        j1 = j0 + ty < y1[r] ? j0 + ty : y1[r];
     }
-    __host__ __device__ void tile(int t, int &bx, int &by) const
+    static RectSet around(int imax, int jmax, int ty, int xa, int xb, int ya, int yb)
     {
-        const int nb = fy0 * nbx;
-        if (t < nb) {
-            by = t / nbx;
-            bx = t - by * nbx;
-            return;
+        RectSet s{};
+        s.ty = ty;
+        if (xb <= xa || yb <= ya) xa = xb = ya = yb = 0;
+        const int r[4][4] = {{0, imax, 0, ya}, {0, imax, yb, jmax}, {0, xa, ya, yb}, {xb, imax, ya, yb}};
+        int n = 0;
+        for (int q = 0; q < 4; ++q) {
+            int X0 = r[q][0], X1 = r[q][1], Y0 = r[q][2], Y1 = r[q][3];
+            if (q == 0 && xa == xb) Y1 = jmax;  // empty inner rectangle: the whole grid
+            s.x0[q] = X0;
+            s.x1[q] = X1;
+            s.y0[q] = Y0;
+            s.y1[q] = Y1;
+            const bool empty = X1 <= X0 || Y1 <= Y0 || (q > 0 && xa == xb);
+            s.tx[q] = empty ? 1 : (X1 - X0 + 63) / 64;
+            s.start[q] = n;
+            if (!empty) n += s.tx[q] * ((Y1 - Y0 + ty - 1) / ty);
         }
-        t -= nb;
-        const int nt = (nby - fy1) * nbx;
-        if (t < nt) {
-            const int r = t / nbx;
-            by = fy1 + r;
-            bx = t - r * nbx;
-            return;
-        }
-        t -= nt;
-        const int wl = fx0, w = fx0 + nbx - fx1;
-        const int r = t / w, c = t - r * w;
-        by = fy0 + r;
-        bx = c < wl ? c : fx1 + (c - wl);
+        s.start[4] = n;
+        return s;
     }
 };
+
+// marching region: output columns [xa, xa + 58 nstrips), rows [ya, yb) in
+// chunks of H rows (blockIdx.x = strip, blockIdx.y = chunk)
+struct MarchMap {
+    int xa, nstrips, ya, yb, H;
+};
+
+// Output columns / rows of the marching region for strips of 58 and steps of
+// rb rows: every output's 3-point level-1 halo inside [max(D,4), n - max(D,4))
+// in both directions, rows a multiple of rb.  False when it is too small.
+__host__ __device__ inline bool march_region(int fast_ok, int D, int imax, int jmax, int rb, MarchMap &m)
+{
+    if (!fast_ok) return false;
+    const int Dm = D > 4 ? D : 4;
+    m.xa = Dm + 3;
+    m.nstrips = (imax - Dm - 3 - m.xa) / 58;
+    m.ya = Dm + 3;
+    const int rows = jmax - Dm - 3 - m.ya;
+    m.yb = m.ya + (rows > 0 ? rows / rb * rb : 0);
+    m.H = 0;
+    return m.nstrips >= 1 && m.yb - m.ya >= 4 * rb;
+}
 
 }  // namespace nsamd

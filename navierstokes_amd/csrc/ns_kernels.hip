@@ -58,38 +58,212 @@ __device__ __forceinline__ R drp_interior_unit(const F &f, int q)
     return (t1 + t2) + t3;
 }
 
-// FAST: the tile and its 3-point level-1 halo lie in the interior, outside
+// ---------------------------------------------------------------------------
+// Per-point pieces of the stage, shared by the tile and the marching kernels,
+// which therefore compute every value with the same operations.
+// FAST: the point and its 3-point level-1 halo lie in the interior, outside
 // the metric/buffer layers, on a unit grid.  Then no one-sided stencil, no
 // metric factor (it is exactly 1), no buffer and no edge test is needed, and
 // dropping the multiplications by 1.0 leaves every result bit unchanged.
+// ---------------------------------------------------------------------------
+template <typename R, bool FAST, typename F>
+__device__ __forceinline__ R dv(const F &f, int q, int n, R inv_h)
+{
+    if constexpr (FAST) return drp_interior_unit<R>(f, q);
+    else return drp_deriv<R>(f, q, n, inv_h);
+}
+
+template <typename R, bool FAST>
+__device__ __forceinline__ R met(R d, R m)
+{
+    if constexpr (FAST) return d;
+    else return d * m;
+}
+
+// level-1 slots: 0..4 = tau_xx, tau_xy, tau_yy, k dT/dx, k dT/dy (viscous),
+// then the species flux rho alpha (psi_x + psi_y) (nk = 5)
+template <int NK, bool VISC>
+struct L1 {
+    static constexpr int PSI = VISC ? 5 : 0;
+    static constexpr int N = (VISC ? 5 : 0) + (NK >= 5 ? 1 : 0);
+};
+
+// Level-1 fields at (gx, gy) (eqeuler.f90:34-41, derivs.f90:25-56, :40).
+// ux(k) / uy(k): accessor of stored component k along x at row gy / along y
+// at column gx; tx / ty: the same for T (T - Tb in the perturbation form);
+// tval: that value at the point.
+template <typename S, typename R, int NK, bool VISC, bool PERT, bool FAST, typename UXF, typename UYF, typename TXF,
+          typename TYF>
+__device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transport<R> &tc, int gx, int gy,
+                                             const UXF &ux, const UYF &uy, const TXF &tx, const TYF &ty, R tval,
+                                             R *out)
+{
+    const int imax = A.imax, jmax = A.jmax;
+    const R inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn);
+    const R mx = FAST ? R(1) : R(A.metx[gx]), my = FAST ? R(1) : R(A.mety[gy]);
+    if constexpr (NK >= 5) {
+        const R psx = met<R, FAST>(dv<R, FAST>(ux(4), gx, imax, inv_dm), mx);
+        R psy = met<R, FAST>(dv<R, FAST>(uy(4), gy, jmax, inv_dn), my);
+        R rho = ux(0)(gx);
+        if constexpr (PERT) {
+            psy = psy + R(A.pr.dub[4][gy]);
+            rho = rho + R(A.pr.ub[0][gy]);
+        }
+        out[L1<NK, VISC>::PSI] = rho * R(A.alpha_cond) * (psx + psy);  // derivs.f90:40
+    }
+    if constexpr (VISC) {
+        const R uxv = met<R, FAST>(dv<R, FAST>(ux(1), gx, imax, inv_dm), mx);
+        const R vxv = met<R, FAST>(dv<R, FAST>(ux(2), gx, imax, inv_dm), mx);
+        R uyv = met<R, FAST>(dv<R, FAST>(uy(1), gy, jmax, inv_dn), my);
+        R vyv = met<R, FAST>(dv<R, FAST>(uy(2), gy, jmax, inv_dn), my);
+        const R Tx = met<R, FAST>(dv<R, FAST>(tx, gx, imax, inv_dm), mx);
+        R Ty = met<R, FAST>(dv<R, FAST>(ty, gy, jmax, inv_dn), my);
+        if constexpr (PERT) {
+            uyv = uyv + R(A.pr.dub[1][gy]);
+            vyv = vyv + R(A.pr.dub[2][gy]);
+            Ty += R(A.pr.dtb[gy]);
+            tval += R(A.pr.tb[gy]);
+        }
+        R mu, kk;
+        transport_eval<R>(tc, tval, mu, kk);
+        const R c23mu = R(2.0 / 3.0) * mu;
+        const R div = uxv + vyv;
+        out[0] = -(c23mu * div) + R(2) * mu * uxv;  // eqeuler.f90:37
+        out[1] = mu * (uyv + vxv);                  // :38
+        out[2] = -(c23mu * div) + R(2) * mu * vyv;  // :39
+        out[3] = Tx * kk;                           // derivs.f90:55
+        out[4] = Ty * kk;                           // :54
+    }
+}
+
+// RHS at (gx, gy) (eqeuler.f90:61-97): f[], the stored state qc[] and the
+// full state uc[] there.  lx(l) / ly(l): accessor of level-1 field l along x
+// at row gy / along y at column gx; lv(l): its value at the point.
+template <typename S, typename R, int NK, bool VISC, bool PERT, bool FAST, typename UXF, typename UYF, typename LXF,
+          typename LYF, typename LVF>
+__device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy, const UXF &ux, const UYF &uy,
+                                          const LXF &lx, const LYF &ly, const LVF &lv, R *qc, R *uc, R *f)
+{
+    const int imax = A.imax, jmax = A.jmax;
+    const R inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn), gam = R(A.gamma);
+    const R mx = FAST ? R(1) : R(A.metx[gx]), my = FAST ? R(1) : R(A.mety[gy]);
+    R fx[NK], fy[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        qc[k] = ux(k)(gx);
+        uc[k] = qc[k];
+        fx[k] = met<R, FAST>(dv<R, FAST>(ux(k), gx, imax, inv_dm), mx);
+        fy[k] = met<R, FAST>(dv<R, FAST>(uy(k), gy, jmax, inv_dn), my);
+        if constexpr (PERT) {
+            uc[k] = qc[k] + R(A.pr.ub[k][gy]);
+            fy[k] = fy[k] + R(A.pr.dub[k][gy]);
+        }
+    }
+    const R rho = uc[0], u = uc[1], v = uc[2], p = uc[3];
+    const R irho = R(1) / rho;
+    const R src = (A.amp_src != 0.0) ? R(A.amp_src * (A.gx[gx] * A.gy[gy])) : R(0);
+    // eqeuler.f90:63-66
+    f[0] = -(u * fx[0] + rho * fx[1] + v * fy[0] + rho * fy[2]);
+    if constexpr (VISC) {
+        const R txx_x = met<R, FAST>(dv<R, FAST>(lx(0), gx, imax, inv_dm), mx);
+        const R txy_x = met<R, FAST>(dv<R, FAST>(lx(1), gx, imax, inv_dm), mx);
+        const R ktx_x = met<R, FAST>(dv<R, FAST>(lx(3), gx, imax, inv_dm), mx);
+        const R txy_y = met<R, FAST>(dv<R, FAST>(ly(1), gy, jmax, inv_dn), my);
+        const R tyy_y = met<R, FAST>(dv<R, FAST>(ly(2), gy, jmax, inv_dn), my);
+        const R kty_y = met<R, FAST>(dv<R, FAST>(ly(4), gy, jmax, inv_dn), my);
+        const R t0 = lv(0), t1 = lv(1), t2 = lv(2);
+        const R vre = R(A.inv_Re) * irho;
+        // eqeuler.f90:68-92
+        f[1] = -(u * fx[1] + irho * fx[3] + v * fy[1] - vre * (txx_x + txy_y));
+        f[2] = -(u * fx[2] + v * fy[2] + irho * fy[3] - vre * (txy_x + tyy_y));
+        f[3] = -(gam * p * fx[1] + u * fx[3] + gam * p * fy[2] + v * fy[3] - src -
+                 R(A.gm1_Re) * (fx[1] * t0 + fx[2] * t1 + fy[1] * t1 + fy[2] * t2) -
+                 R(A.inv_RePr) * (ktx_x + kty_y));
+    } else {
+        f[1] = -(u * fx[1] + irho * fx[3] + v * fy[1]);
+        f[2] = -(u * fx[2] + v * fy[2] + irho * fy[3]);
+        f[3] = -(gam * p * fx[1] + u * fx[3] + gam * p * fy[2] + v * fy[3] - src);
+    }
+    if constexpr (NK >= 5) {
+        constexpr int P = L1<NK, VISC>::PSI;
+        const R psd_x = met<R, FAST>(dv<R, FAST>(lx(P), gx, imax, inv_dm), mx);
+        const R psd_y = met<R, FAST>(dv<R, FAST>(ly(P), gy, jmax, inv_dn), my);
+        f[4] = -(u * fx[4] + v * fy[4] - R(A.inv_Pe) * irho * (psd_x + psd_y));  // eqeuler.f90:94-95
+    }
+}
+
+// LDDRK update at interior points (rk.f90:160-167) and the buffer layers
+// (rk.f90:81-108), or the RHS alone (RHS_ONLY).  wpre: W at the point, loaded
+// ahead by the caller (null: load it here).
+template <typename S, typename R, int NK, bool RHS_ONLY, bool PERT, bool FAST>
+__device__ __forceinline__ void stage_store(const StageArgs<S> &A, int gx, int gy, const R *qc, const R *uc,
+                                            const R *f, const S *wpre = nullptr)
+{
+    const int imax = A.imax, jmax = A.jmax;
+    const size_t g = (size_t)gy * A.pitch + gx;
+    if constexpr (RHS_ONLY) {
+#pragma unroll
+        for (int k = 0; k < NK; ++k) A.dudt[k][g] = S(f[k]);
+        return;
+    }
+    const bool interior = FAST || (gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2);
+    const bool ybuf = !FAST && (gy < A.D || gy >= jmax - A.D);
+    const bool xbuf = !FAST && (gx < A.D || gx >= imax - A.D);
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        if constexpr (PERT) {
+            // stage accumulation of the state in FP64 registers
+            double qn = double(qc[k]);
+            if (interior) {
+                const double w0 = double(wpre ? wpre[k] : A.W[k][g]);
+                const double wn = A.first ? A.dt * double(f[k]) : A.alpha * w0 + A.dt * double(f[k]);
+                if (!A.last) A.W[k][g] = S(wn);
+                qn = qn + A.beta * wn;
+            }
+            // Ub + s (U - Ub) - Ub = s q
+            if (ybuf) qn = A.sigy[gy] * qn;
+            if (xbuf) qn = A.sigx[gx] * qn;
+            A.Uout[k][g] = S(qn);
+        } else {
+            R un = uc[k];
+            if (interior) {
+                const R w0 = R(wpre ? wpre[k] : A.W[k][g]);
+                const R wn = A.first ? R(A.dt) * f[k] : R(A.alpha) * w0 + R(A.dt) * f[k];
+                if (!A.last) A.W[k][g] = S(wn);
+                un = uc[k] + R(A.beta) * wn;
+            }
+            if (ybuf || xbuf) {  // y layer first, then x
+                const R ub = R(A.base[k][g]);
+                if (ybuf) un = ub + R(A.sigy[gy]) * (un - ub);
+                if (xbuf) un = ub + R(A.sigx[gx]) * (un - ub);
+            }
+            A.Uout[k][g] = S(un);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile version: one block computes the outputs [i0,i1) x [j0,j1) (at most
+// STX x TY) from an LDS box holding U (and T) on the tile plus a 6-point halo
+// and the level-1 fields on the tile plus a 3-point halo.
+// ---------------------------------------------------------------------------
 template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT, bool FAST>
-__device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int bx, int by)
+__device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1, int j0, int j1)
 {
     constexpr int TX = STX;
     constexpr int WX = TX + 12, WY = TY + 12;
     constexpr int LX = TX + 6, LY = TY + 6;
+    constexpr int NL = L1<NK, VISC>::N;
     extern __shared__ char smem_raw[];
     R *sU = reinterpret_cast<R *>(smem_raw);  // NK x WY x WX   (U, or q in the perturbation form)
     R *sT = sU + NK * WY * WX;                // WY x WX (VISC)  (T, or T - Tb)
     R *sL = sT + (VISC ? WY * WX : 0);        // level-1 fields: LY x LX each
-    constexpr bool SPECIES = NK >= 5;          // passive scalar: Pe diffusion (independent of Re)
-    constexpr int LPSI = VISC ? 5 : 0;         // level-1 slot of the species flux
 
     const int imax = A.imax, jmax = A.jmax, pitch = A.pitch;
-    const int i0 = bx * TX, j0 = by * TY;
-    const int i1 = min(i0 + TX, imax), j1 = min(j0 + TY, jmax);
     const Range rx = tile_ranges(i0, i1, imax);
     const Range ry = tile_ranges(j0, j1, jmax);
     const int tid = threadIdx.x;
-    const R gam = R(A.gamma), inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn);
-    auto DV = [&](const auto &f, int q, int n, R inv_h) -> R {
-        if constexpr (FAST) return drp_interior_unit<R>(f, q);
-        else return drp_deriv<R>(f, q, n, inv_h);
-    };
-    auto MET = [&](R d, R m) -> R {
-        if constexpr (FAST) return d;
-        else return d * m;
-    };
+    const R gam = R(A.gamma);
 
     // phase 1: U window (and T = gamma p / rho, rhs.f90:20)
     const int wxn = rx.w1 - rx.w0, wyn = ry.w1 - ry.w0;
@@ -115,26 +289,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int bx, int by
     __syncthreads();
 
     const int ox = rx.w0, oy = ry.w0;
-    auto UX = [&](int k, int gy) {  // accessor along x at global row gy
-        const R *row = sU + (k * WY + (gy - oy)) * WX;
-        return [row, ox](int q) { return row[q - ox]; };
+    auto ux_at = [=](int gy) {  // component accessors along x at global row gy
+        return [=](int k) {
+            const R *row = sU + (k * WY + (gy - oy)) * WX;
+            return [row, ox](int q) { return row[q - ox]; };
+        };
     };
-    auto UYc = [&](int k, int gx) {  // accessor along y at global column gx
-        const R *col = sU + k * WY * WX + (gx - ox);
-        return [col, oy](int q) { return col[(q - oy) * WX]; };
-    };
-    // base-flow y-derivative (perturbation form), 0 otherwise
-    auto dbase = [&](int k, int gy) -> R {
-        if constexpr (PERT) return R(A.pr.dub[k][gy]);
-        return R(0);
-    };
-    auto base = [&](int k, int gy) -> R {
-        if constexpr (PERT) return R(A.pr.ub[k][gy]);
-        return R(0);
+    auto uy_at = [=](int gx) {  // component accessors along y at global column gx
+        return [=](int k) {
+            const R *col = sU + k * WY * WX + (gx - ox);
+            return [col, oy](int q) { return col[(q - oy) * WX]; };
+        };
     };
 
     // phase 2: level-1 quantities (viscous stresses, k grad T, species flux)
-    if constexpr (VISC || SPECIES) {
+    if constexpr (NL > 0) {
         const Transport<R> tc{R(A.tc.A_mi), R(A.tc.Bt_mi), R(A.tc.Ct_mi), R(A.tc.D_mi), R(A.tc.inv_mi0),
                               R(A.tc.A_k),  R(A.tc.Bt_k),  R(A.tc.Ct_k),  R(A.tc.D_k),  R(A.tc.inv_k0),
                               R(A.tc.Tref)};
@@ -143,153 +312,54 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int bx, int by
             const int ly = idx / LX, lx = idx - ly * LX;
             if (lx >= lxn) continue;
             const int gx = rx.l0 + lx, gy = ry.l0 + ly;
-            const R mx = R(A.metx[gx]), my = R(A.mety[gy]);
-            const int o = ly * LX + lx;
-            if constexpr (SPECIES) {
-                const R psx = MET(DV(UX(4, gy), gx, imax, inv_dm), mx);
-                const R psy = MET(DV(UYc(4, gx), gy, jmax, inv_dn), my) + dbase(4, gy);
-                const R rho = UX(0, gy)(gx) + base(0, gy);
-                sL[LPSI * LY * LX + o] = rho * R(A.alpha_cond) * (psx + psy);  // derivs.f90:40
-            }
-            if constexpr (!VISC) continue;
-            const R ux = MET(DV(UX(1, gy), gx, imax, inv_dm), mx);
-            const R vx = MET(DV(UX(2, gy), gx, imax, inv_dm), mx);
-            const R uy = MET(DV(UYc(1, gx), gy, jmax, inv_dn), my) + dbase(1, gy);
-            const R vy = MET(DV(UYc(2, gx), gy, jmax, inv_dn), my) + dbase(2, gy);
             const R *trow = sT + (gy - oy) * WX;
             const R *tcol = sT + (gx - ox);
-            const R Tx = MET(DV([trow, ox](int q) { return trow[q - ox]; }, gx, imax, inv_dm), mx);
-            R Ty = MET(DV([tcol, oy](int q) { return tcol[(q - oy) * WX]; }, gy, jmax, inv_dn), my);
-            R Tval = trow[gx - ox];
-            if constexpr (PERT) {
-                Ty += R(A.pr.dtb[gy]);
-                Tval += R(A.pr.tb[gy]);
-            }
-            R mu, kk;
-            transport_eval<R>(tc, Tval, mu, kk);
-            const R c23mu = R(2.0 / 3.0) * mu;
-            const R div = ux + vy;
-            sL[0 * LY * LX + o] = -(c23mu * div) + R(2) * mu * ux;  // eqeuler.f90:37
-            sL[1 * LY * LX + o] = mu * (uy + vx);                    // :38
-            sL[2 * LY * LX + o] = -(c23mu * div) + R(2) * mu * vy;  // :39
-            sL[3 * LY * LX + o] = Tx * kk;                           // derivs.f90:55
-            sL[4 * LY * LX + o] = Ty * kk;                           // :54
+            R out[NL];
+            level1_point<S, R, NK, VISC, PERT, FAST>(
+                A, tc, gx, gy, ux_at(gy), uy_at(gx), [trow, ox](int q) { return trow[q - ox]; },
+                [tcol, oy](int q) { return tcol[(q - oy) * WX]; }, VISC ? trow[gx - ox] : R(0), out);
+            const int o = ly * LX + lx;
+#pragma unroll
+            for (int l = 0; l < NL; ++l) sL[l * LY * LX + o] = out[l];
         }
         __syncthreads();
     }
 
     // phase 3: RHS at the tile's points, then the stage update
-    const int tx = tid & 63;
-    const int gx = i0 + tx;
+    const int gx = i0 + (tid & 63);
     if (gx >= i1) return;
-    const R mx = R(A.metx[gx]);
+    const int lx0 = rx.l0, ly0 = ry.l0;
     for (int gy = j0 + (tid >> 6); gy < j1; gy += NT / 64) {
-        const bool interior = FAST || (gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2);
-        const R my = R(A.mety[gy]);
-        R qc[NK], uc[NK], fx[NK], fy[NK];
-#pragma unroll
-        for (int k = 0; k < NK; ++k) {
-            qc[k] = UX(k, gy)(gx);
-            uc[k] = qc[k] + base(k, gy);
-            fx[k] = MET(DV(UX(k, gy), gx, imax, inv_dm), mx);
-            fy[k] = MET(DV(UYc(k, gx), gy, jmax, inv_dn), my) + dbase(k, gy);
-        }
-        const R rho = uc[0], u = uc[1], v = uc[2], p = uc[3];
-        const R irho = R(1) / rho;
-        const R src = (A.amp_src != 0.0) ? R(A.amp_src * (A.gx[gx] * A.gy[gy])) : R(0);
-        R f[NK];
-        // eqeuler.f90:63-66
-        f[0] = -(u * fx[0] + rho * fx[1] + v * fy[0] + rho * fy[2]);
-        if constexpr (VISC) {
-            const int lx0 = rx.l0, ly0 = ry.l0;
-            auto LXa = [&](int l) {
-                const R *row = sL + (l * LY + (gy - ly0)) * LX;
-                return [row, lx0](int q) { return row[q - lx0]; };
-            };
-            auto LYa = [&](int l) {
-                const R *col = sL + l * LY * LX + (gx - lx0);
-                return [col, ly0](int q) { return col[(q - ly0) * LX]; };
-            };
-            const R txx_x = MET(DV(LXa(0), gx, imax, inv_dm), mx);
-            const R txy_x = MET(DV(LXa(1), gx, imax, inv_dm), mx);
-            const R ktx_x = MET(DV(LXa(3), gx, imax, inv_dm), mx);
-            const R txy_y = MET(DV(LYa(1), gy, jmax, inv_dn), my);
-            const R tyy_y = MET(DV(LYa(2), gy, jmax, inv_dn), my);
-            const R kty_y = MET(DV(LYa(4), gy, jmax, inv_dn), my);
-            const int o = (gy - ly0) * LX + (gx - lx0);
-            const R t0 = sL[0 * LY * LX + o], t1 = sL[1 * LY * LX + o], t2 = sL[2 * LY * LX + o];
-            const R vre = R(A.inv_Re) * irho;
-            // eqeuler.f90:68-92
-            f[1] = -(u * fx[1] + irho * fx[3] + v * fy[1] - vre * (txx_x + txy_y));
-            f[2] = -(u * fx[2] + v * fy[2] + irho * fy[3] - vre * (txy_x + tyy_y));
-            f[3] = -(gam * p * fx[1] + u * fx[3] + gam * p * fy[2] + v * fy[3] - src -
-                     R(A.gm1_Re) * (fx[1] * t0 + fx[2] * t1 + fy[1] * t1 + fy[2] * t2) -
-                     R(A.inv_RePr) * (ktx_x + kty_y));
-        } else {
-            f[1] = -(u * fx[1] + irho * fx[3] + v * fy[1]);
-            f[2] = -(u * fx[2] + v * fy[2] + irho * fy[3]);
-            f[3] = -(gam * p * fx[1] + u * fx[3] + gam * p * fy[2] + v * fy[3] - src);
-        }
-        if constexpr (SPECIES) {
-            const int lx0 = rx.l0, ly0 = ry.l0;
-            const R *prow = sL + (LPSI * LY + (gy - ly0)) * LX;
-            const R *pcol = sL + LPSI * LY * LX + (gx - lx0);
-            const R psd_x = MET(DV([prow, lx0](int q) { return prow[q - lx0]; }, gx, imax, inv_dm), mx);
-            const R psd_y = MET(DV([pcol, ly0](int q) { return pcol[(q - ly0) * LX]; }, gy, jmax, inv_dn), my);
-            f[4] = -(u * fx[4] + v * fy[4] - R(A.inv_Pe) * irho * (psd_x + psd_y));  // eqeuler.f90:94-95
-        }
-        const size_t g = (size_t)gy * pitch + gx;
-        if constexpr (RHS_ONLY) {
-#pragma unroll
-            for (int k = 0; k < NK; ++k) A.dudt[k][g] = S(f[k]);
-        } else {
-            const bool ybuf = !FAST && (gy < A.D || gy >= jmax - A.D);
-            const bool xbuf = !FAST && (gx < A.D || gx >= imax - A.D);
-#pragma unroll
-            for (int k = 0; k < NK; ++k) {
-                if constexpr (PERT) {
-                    // stage accumulation of the state in FP64 registers
-                    double qn = double(qc[k]);
-                    if (interior) {  // rk.f90:160-167
-                        const double wn = A.first ? A.dt * double(f[k]) : A.alpha * double(A.W[k][g]) + A.dt * double(f[k]);
-                        if (!A.last) A.W[k][g] = S(wn);
-                        qn = qn + A.beta * wn;
-                    }
-                    // Ub + s (U - Ub) - Ub = s q  (rk.f90:81-108)
-                    if (ybuf) qn = A.sigy[gy] * qn;
-                    if (xbuf) qn = A.sigx[gx] * qn;
-                    A.Uout[k][g] = S(qn);
-                } else {
-                    R un = uc[k];
-                    if (interior) {  // rk.f90:160-167
-                        const R wn = A.first ? R(A.dt) * f[k] : R(A.alpha) * R(A.W[k][g]) + R(A.dt) * f[k];
-                        if (!A.last) A.W[k][g] = S(wn);
-                        un = uc[k] + R(A.beta) * wn;
-                    }
-                    if (ybuf || xbuf) {  // rk.f90:81-108 (y layer first, then x)
-                        const R ub = R(A.base[k][g]);
-                        if (ybuf) un = ub + R(A.sigy[gy]) * (un - ub);
-                        if (xbuf) un = ub + R(A.sigx[gx]) * (un - ub);
-                    }
-                    A.Uout[k][g] = S(un);
-                }
-            }
-        }
+        auto lxa = [=](int l) {
+            const R *row = sL + (l * LY + (gy - ly0)) * LX;
+            return [row, lx0](int q) { return row[q - lx0]; };
+        };
+        auto lya = [=](int l) {
+            const R *col = sL + l * LY * LX + (gx - lx0);
+            return [col, ly0](int q) { return col[(q - ly0) * LX]; };
+        };
+        auto lv = [=](int l) { return sL[l * LY * LX + (gy - ly0) * LX + (gx - lx0)]; };
+        R qc[NK], uc[NK], f[NK];
+        rhs_point<S, R, NK, VISC, PERT, FAST>(A, gx, gy, ux_at(gy), uy_at(gx), lxa, lya, lv, qc, uc, f);
+        stage_store<S, R, NK, RHS_ONLY, PERT, FAST>(A, gx, gy, qc, uc, f);
     }
 }
 
-template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT, bool FAST>
-__global__ __launch_bounds__(NT) void stage_kernel(StageArgs<S> A, int bx0, int by0)
+// interior tiles of the fast rectangle (tile indices from bx0, by0)
+template <typename S, typename R, int NK, bool VISC, int TY, bool PERT, int NT>
+__global__ __launch_bounds__(NT) void stage_kernel_fast(StageArgs<S> A, int bx0, int by0)
 {
-    stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, FAST>(A, bx0 + blockIdx.x, by0 + blockIdx.y);
+    const int i0 = (bx0 + (int)blockIdx.x) * STX, j0 = (by0 + (int)blockIdx.y) * TY;
+    stage_tile<S, R, NK, VISC, TY, false, PERT, NT, true>(A, i0, i0 + STX, j0, j0 + TY);
 }
 
+// general tiles over up to four rectangles (the grid around the interior region)
 template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT>
-__global__ __launch_bounds__(NT) void stage_kernel_ring(StageArgs<S> A, RingMap m)
+__global__ __launch_bounds__(NT) void stage_kernel_rects(StageArgs<S> A, RectSet rs)
 {
-    int bx, by;
-    m.tile(blockIdx.x, bx, by);
-    stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, false>(A, bx, by);
+    int i0, i1, j0, j1;
+    rs.tile(blockIdx.x, i0, i1, j0, j1);
+    stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, false>(A, i0, i1, j0, j1);
 }
 
 // Batched launch: one block per entry of a tile table spanning every case;
@@ -301,7 +371,182 @@ __global__ __launch_bounds__(NT) void stage_kernel_batch(const StageArgs<S> *arg
                                                          int ncases)
 {
     const int4 t = tiles[blockIdx.x];
-    stage_tile<S, R, NK, VISC, TY, false, PERT, NT, FAST>(args[(size_t)stage * ncases + t.x], t.y, t.z);
+    const StageArgs<S> &A = args[(size_t)stage * ncases + t.x];
+    const int i0 = t.y * STX, j0 = t.z * TY;
+    stage_tile<S, R, NK, VISC, TY, false, PERT, NT, FAST>(A, i0, min(i0 + STX, A.imax), j0, min(j0 + TY, A.jmax));
+}
+
+// ---------------------------------------------------------------------------
+// Marching version (interior region only).  A block owns a strip of MTX = 58
+// output columns, so its level-1 rows are exactly 64 wide (one wave per row)
+// and its U window rows 70 wide, and marches up a chunk of rows RB at a time
+// (RB waves).  Window rows and level-1 rows live in LDS rings: every row is
+// loaded, and every level-1 row computed, once per chunk.  Per step:
+//   (a) store the RB window rows prefetched during the previous step (+ T),
+//       barrier, issue the loads of the next step's rows into registers;
+//   (b) wave w computes level-1 row base+3+w, barrier;
+//   (c) wave w computes the outputs of row base+w.
+// The window ring holds 2 RB + 9 rows: (c) reads rows base-3 .. base+RB+2,
+// (b) rows up to base+RB+5, and (a) of the next step may already store rows
+// base+RB+6 .. base+2RB+5 while other waves are still in (c).  The level-1
+// ring holds RB + 6 rows: (b) of the next step runs only after the barrier
+// that follows (a), when every wave has left (c).  Row indices are wave-uniform, so the ring slot
+// arithmetic runs on the scalar unit.
+// ---------------------------------------------------------------------------
+constexpr int MTX = 58, MLX = 64, MWX = 70;
+
+template <int RB>
+struct MarchRing {
+    static constexpr int WR = 2 * RB + 9;
+    static constexpr int LR = RB + 6;
+};
+
+template <typename R>
+static size_t march_smem(int NK, bool visc, int RB)
+{
+    const int WR = 2 * RB + 9, LR = RB + 6;
+    const int NL = (visc ? 5 : 0) + (NK >= 5 ? 1 : 0);
+    return sizeof(R) * ((size_t)(NK + (visc ? 1 : 0)) * WR * MWX + (size_t)NL * LR * MLX);
+}
+
+template <typename S, typename R, int NK, bool VISC, bool PERT, int RB>
+__global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, MarchMap m)
+{
+    constexpr int NT = RB * 64, WR = MarchRing<RB>::WR, LR = MarchRing<RB>::LR;
+    constexpr int NL = L1<NK, VISC>::N;
+    constexpr int PF = (RB * MWX + NT - 1) / NT;  // prefetched window points per thread
+    extern __shared__ char smem_raw[];
+    R *sU = reinterpret_cast<R *>(smem_raw);  // NK x WR x MWX
+    R *sT = sU + NK * WR * MWX;               // WR x MWX (VISC)
+    R *sL = sT + (VISC ? WR * MWX : 0);       // NL x LR x MLX
+
+    const int pitch = A.pitch;
+    const int i0 = m.xa + (int)blockIdx.x * MTX;
+    const int y0 = m.ya + (int)blockIdx.y * m.H;
+    const int y1 = min(y0 + m.H, m.yb);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ox = i0 - 6, lx0 = i0 - 3;  // window / level-1 column origins
+    const int wy0 = y0 - 6, ly0 = y0 - 3;  // first window / level-1 row of the chunk
+    const R gam = R(A.gamma);
+
+    auto load_pt = [&](int r, int c, R *v) {
+        const size_t g = (size_t)r * pitch + (ox + c);
+#pragma unroll
+        for (int k = 0; k < NK; ++k) v[k] = R(A.U[k][g]);
+    };
+    auto put_pt = [&](int r, int c, const R *v) {
+        const int s = (r - wy0) % WR;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) sU[(k * WR + s) * MWX + c] = v[k];
+        if constexpr (VISC) {
+            if constexpr (PERT) {
+                const R rb = R(A.pr.ub[0][r]), pb = R(A.pr.ub[3][r]);
+                sT[s * MWX + c] = gam * (v[3] * rb - pb * v[0]) / ((rb + v[0]) * rb);
+            } else {
+                sT[s * MWX + c] = gam * v[3] / v[0];
+            }
+        }
+    };
+    // accessors (r, gy: wave-uniform rows)
+    auto ux_at = [=](int r) {
+        const int s = (r - wy0) % WR;
+        return [=](int k) {
+            const R *row = sU + (k * WR + s) * MWX;
+            return [row, ox](int q) { return row[q - ox]; };
+        };
+    };
+    auto uy_at = [=](int gx) {
+        return [=](int k) {
+            const R *col = sU + k * WR * MWX + (gx - ox);
+            return [col, wy0](int q) { return col[((q - wy0) % WR) * MWX]; };
+        };
+    };
+    auto tx_at = [=](int r) {
+        const R *row = sT + ((r - wy0) % WR) * MWX;
+        return [row, ox](int q) { return row[q - ox]; };
+    };
+    auto ty_at = [=](int gx) {
+        const R *col = sT + (gx - ox);
+        return [col, wy0](int q) { return col[((q - wy0) % WR) * MWX]; };
+    };
+    const Transport<R> tc{R(A.tc.A_mi), R(A.tc.Bt_mi), R(A.tc.Ct_mi), R(A.tc.D_mi), R(A.tc.inv_mi0),
+                          R(A.tc.A_k),  R(A.tc.Bt_k),  R(A.tc.Ct_k),  R(A.tc.D_k),  R(A.tc.inv_k0),
+                          R(A.tc.Tref)};
+    auto level1_row = [&](int r) {  // wave-uniform row r, column lx0 + lane
+        const int gx = lx0 + lane;
+        R out[NL > 0 ? NL : 1];
+        level1_point<S, R, NK, VISC, PERT, true>(A, tc, gx, r, ux_at(r), uy_at(gx), tx_at(r), ty_at(gx),
+                                                  VISC ? tx_at(r)(gx) : R(0), out);
+        const int s = (r - ly0) % LR;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) sL[(l * LR + s) * MLX + lane] = out[l];
+    };
+
+    // prologue: window rows y0-6 .. y0+5, level-1 rows y0-3 .. y0+2
+    for (int t = tid; t < 12 * MWX; t += NT) {
+        const int r = wy0 + t / MWX, c = t % MWX;
+        R v[NK];
+        load_pt(r, c, v);
+        put_pt(r, c, v);
+    }
+    __syncthreads();
+    if constexpr (NL > 0) {
+        for (int rr = wave; rr < 6; rr += RB) level1_row(ly0 + rr);
+    }
+    R pre[PF][NK];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+        const int t = tid + p * NT;
+        if (t < RB * MWX) load_pt(y0 + 6 + t / MWX, t % MWX, pre[p]);
+    }
+
+    for (int base = y0; base < y1; base += RB) {
+        // W of this step's outputs, loaded now so that the loads overlap (a) and (b)
+        S wpre[NK];
+        {
+            const size_t g = (size_t)(base + wave) * pitch + (i0 + lane);
+#pragma unroll
+            for (int k = 0; k < NK; ++k) wpre[k] = (lane < MTX && !A.first) ? A.W[k][g] : S(0);
+        }
+        // (a) window rows base+6 .. base+RB+5
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            const int t = tid + p * NT;
+            if (t < RB * MWX) put_pt(base + 6 + t / MWX, t % MWX, pre[p]);
+        }
+        __syncthreads();
+        if (base + RB < y1) {
+#pragma unroll
+            for (int p = 0; p < PF; ++p) {
+                const int t = tid + p * NT;
+                if (t < RB * MWX) load_pt(base + RB + 6 + t / MWX, t % MWX, pre[p]);
+            }
+        }
+        // (b) level-1 row base+3+wave
+        if constexpr (NL > 0) {
+            level1_row(base + 3 + wave);
+            __syncthreads();
+        }
+        // (c) outputs of row base+wave
+        const int gy = base + wave;
+        if (lane < MTX) {
+            const int gx = i0 + lane;
+            const int ls = (gy - ly0) % LR;
+            auto lxa = [=](int l) {
+                const R *row = sL + (l * LR + ls) * MLX;
+                return [row, lx0](int q) { return row[q - lx0]; };
+            };
+            auto lya = [=](int l) {
+                const R *col = sL + l * LR * MLX + (gx - lx0);
+                return [col, ly0](int q) { return col[((q - ly0) % LR) * MLX]; };
+            };
+            auto lv = [=](int l) { return sL[(l * LR + ls) * MLX + (gx - lx0)]; };
+            R qc[NK], uc[NK], f[NK];
+            rhs_point<S, R, NK, VISC, PERT, true>(A, gx, gy, ux_at(gy), uy_at(gx), lxa, lya, lv, qc, uc, f);
+            stage_store<S, R, NK, false, PERT, true>(A, gx, gy, qc, uc, f, wpre);
+        }
+    }
 }
 
 template <typename R>
@@ -312,52 +557,120 @@ static size_t stage_smem(int NK, bool visc, int TY)
     return sizeof(R) * ((size_t)NK * WY * WX + (visc ? (size_t)WY * WX : 0) + (size_t)NL * LY * LX);
 }
 
-template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, bool FAST>
-static hipError_t launch_stage_rect(const StageArgs<S> &A, int bx0, int bx1, int by0, int by1, hipStream_t s)
+// The marching region: output columns [xa, xa + 58 n) and rows [ya, yb),
+// every point's 3-point level-1 halo inside the fast zone (interior, outside
+// the buffer/metric layers), rows a multiple of RB; chunk height H from a
+// simple occupancy model (blocks per wave of the GPU x rows per block).
+template <typename S, int NK, bool VISC, bool PERT, int RB>
+static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
 {
-    using R = S;
-    constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
-    if (bx1 <= bx0 || by1 <= by0) return hipSuccess;
-    size_t smem = stage_smem<R>(NK, VISC, TY);
-    auto kern = stage_kernel<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, FAST>;
-    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(bx1 - bx0, by1 - by0), dim3(NT), smem, s, A, bx0, by0);
-    return hipGetLastError();
+    if (!march_region(A.fast_ok, A.D, A.imax, A.jmax, RB, m)) return false;
+    const int rows = m.yb - m.ya;
+    static int ncu = 0, perblk = 0;
+    if (!ncu) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (ncu <= 0) ncu = 256;
+        auto kern = stage_march_kernel<S, S, NK, VISC, PERT, RB>;
+        const size_t smem = march_smem<S>(NK, VISC, RB);
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perblk, kern, RB * 64, smem) != hipSuccess || perblk < 1)
+            perblk = 1;
+    }
+    const long slots = (long)ncu * perblk;
+    long best = -1;
+    for (int nc = 1; nc <= rows / (4 * RB); ++nc) {
+        const int H = ((rows + nc - 1) / nc + RB - 1) / RB * RB;
+        const int n = (rows + H - 1) / H;
+        const long blocks = (long)m.nstrips * n;
+        const long cost = ((blocks + slots - 1) / slots) * (H + 8);
+        if (best < 0 || cost < best || (cost == best && n > nchunks)) {
+            best = cost;
+            m.H = H;
+            nchunks = n;
+        }
+    }
+    return true;
 }
 
-// interior rectangle on the fast kernel (stream s), the ring of edge tiles on
-// the general one in a single launch (stream s2 when given, so the two run
-// concurrently; the caller orders s2 after s and joins it back)
-template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT>
+// The stage: the interior region on stream s -- the marching kernel (FP64) or
+// fast tiles (FP32: the tile kernel is faster there, and keeps single cases
+// bitwise equal to the batched path, which uses tiles) -- and the general
+// tile kernel on the rest of the grid in one launch (stream s2 when given, so
+// the two run concurrently; the caller orders s2 after s and joins it back).
+// The RHS-only mode runs the general tiles everywhere.
+template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, int RB>
 static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream_t s2)
 {
     using R = S;
     constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
-    const int nbx = (A.imax + STX - 1) / STX, nby = (A.jmax + TY - 1) / TY;
-    int fx0, fx1, fy0, fy1;
-    fast_range(A.fast_ok, A.D, A.imax, STX, fx0, fx1);
-    fast_range(A.fast_ok, A.D, A.jmax, TY, fy0, fy1);
-    if (fx1 <= fx0 || fy1 <= fy0) return launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, false>(A, 0, nbx, 0, nby, s);
-    const RingMap m{nbx, nby, fx0, fx1, fy0, fy1};
+    constexpr bool MARCH = sizeof(S) == 8;
+    MarchMap m{};
+    int nchunks = 0;
+    int fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
+    bool inner = false;
+    RectSet rs;
+    if constexpr (MARCH) {
+        inner = !RHS_ONLY && march_plan<S, NK, VISC, PERT, RB>(A, m, nchunks);
+        rs = inner ? RectSet::around(A.imax, A.jmax, TY, m.xa, m.xa + m.nstrips * MTX, m.ya, m.yb)
+                   : RectSet::around(A.imax, A.jmax, TY, 0, 0, 0, 0);
+    } else {
+        fast_range(RHS_ONLY ? 0 : A.fast_ok, A.D, A.imax, STX, fx0, fx1);
+        fast_range(RHS_ONLY ? 0 : A.fast_ok, A.D, A.jmax, TY, fy0, fy1);
+        inner = fx1 > fx0 && fy1 > fy0;
+        rs = inner ? RectSet::around(A.imax, A.jmax, TY, fx0 * STX, fx1 * STX, fy0 * TY, fy1 * TY)
+                   : RectSet::around(A.imax, A.jmax, TY, 0, 0, 0, 0);
+    }
+    hipError_t e;
     const size_t smem = stage_smem<R>(NK, VISC, TY);
-    auto ring = stage_kernel_ring<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT>;
-    hipError_t e = hipFuncSetAttribute((const void *)ring, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ring, dim3(m.count()), dim3(NT), smem, s2 ? s2 : s, A, m);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    return launch_stage_rect<S, NK, VISC, RHS_ONLY, PERT, NT, true>(A, fx0, fx1, fy0, fy1, s);
+    auto gen = stage_kernel_rects<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT>;
+    if ((e = hipFuncSetAttribute((const void *)gen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
+        hipSuccess)
+        return e;
+    if (rs.count() > 0) {
+        hipLaunchKernelGGL(gen, dim3(rs.count()), dim3(NT), smem, (inner && s2) ? s2 : s, A, rs);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (!inner) return hipSuccess;
+    if constexpr (!RHS_ONLY && MARCH) {
+        auto kern = stage_march_kernel<S, R, NK, VISC, PERT, RB>;
+        const size_t msm = march_smem<R>(NK, VISC, RB);
+        if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)msm)) !=
+            hipSuccess)
+            return e;
+        hipLaunchKernelGGL(kern, dim3(m.nstrips, nchunks), dim3(RB * 64), msm, s, A, m);
+        return hipGetLastError();
+    } else if constexpr (!RHS_ONLY) {
+        auto kern = stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>;
+        if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
+            hipSuccess)
+            return e;
+        hipLaunchKernelGGL(kern, dim3(fx1 - fx0, fy1 - fy0), dim3(NT), smem, s, A, fx0, fy0);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
-// block-size variants of the stage kernel (same tile, same arithmetic):
-// 0 -> 256 threads, 1 -> 512, 2 -> 1024
+// variants (same arithmetic, bitwise identical results):
+//   FP64: 0 -> marching 4 rows/step, general tiles 256 threads;
+//         1 -> 4 rows, 512; 2 -> 8 rows/step, 512
+//   FP32: tiles of 256, 512 or 1024 threads
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
 static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s, hipStream_t s2)
 {
-    switch (variant) {
-    case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512>(A, s, s2);
-    case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 1024>(A, s, s2);
-    default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 256>(A, s, s2);
+    if constexpr (sizeof(S) == 8) {
+        switch (variant) {
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4>(A, s, s2);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8>(A, s, s2);
+        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 256, 4>(A, s, s2);
+        }
+    } else {
+        switch (variant) {
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4>(A, s, s2);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 1024, 4>(A, s, s2);
+        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 256, 4>(A, s, s2);
+        }
     }
 }
 
