@@ -59,6 +59,21 @@ def make_case(args):
     return case, desc
 
 
+def profiled_traffic(workload: str, size: int, precision: str):
+    """HBM bytes per stage launch measured with rocprofv3 --pmc (FETCH_SIZE
+    x2 + WRITE_SIZE, calibrated on a plain copy), from the committed summary
+    profiles/hbm_traffic_ns4096.md; None when that configuration has none."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_ns4096.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 5 and f[0] == "traffic" and f[1] == workload and int(f[2]) == size and f[3] == precision:
+                return int(f[4])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(case, steps: int):
     """The oracle (C restatement of the reference, OpenMP over host cores)
     timed on a bounded sample of the same workload."""
@@ -331,8 +346,9 @@ def main():
                    "viscous": not case.inviscid, "parallelism": "single-gpu", "stage_variant": variant,
                    "stage_variant_ms": [round(x, 4) for x in tune_ms]},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "stage_kernel (RHS+LDDRK update+buffer)",
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(args.workload, case.imax, args.precision),
+                     "kernel": "stage (RHS+LDDRK update+buffer: interior kernel with the edge tiles beside it)",
                      "alg_bytes_per_launch": B_stage * case.npoints,
                      "avg_launch_ms": round(kt["rhs"], 4)},
         "step_roofline": {"alg_bytes_per_point_step": B_step, "achieved_GBs": round(step_ach, 1),

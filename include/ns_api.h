@@ -186,11 +186,13 @@ int32_t ns_vjp_ccpml(ns_ctx *ctx, double *lam);
 int32_t ns_vjp_step(ns_ctx *ctx, int32_t iter, const double *U, double *lam);
 
 /* tuning ---------------------------------------------------------------------- */
-/* Time the stage kernel's block-size variants (256, 512, 1024 threads on the
-   same LDS tile; results are bitwise identical) on this context's workload and
-   keep the fastest.  Runs by itself before the first step; call it between
-   steps only.  variant (may be NULL) gets the choice 0..2, ms[3] (may be
-   NULL) the average time of each variant (-1 if it could not launch).
+/* Time the stage launch's variants on this context's workload and keep the
+   fastest; results are bitwise identical across variants.  FP64: the
+   interior marching kernel at 4 or 8 rows per step, the edge tiles beside it
+   on a second stream or after it; FP32: interior tiles of 256, 512 or 1024
+   threads.  Runs by itself before the first step; call it between steps
+   only.  variant (may be NULL) gets the choice 0..2, ms[3] (may be NULL) the
+   average time of each variant (-1 if it could not launch).
    No reference counterpart: the reference has no tunable kernels. */
 int32_t ns_autotune(ns_ctx *ctx, int32_t *variant, double *ms);
 

@@ -121,6 +121,7 @@ struct ns_ctx {
     int *dflag = nullptr;
     bool have_profiles = false, have_base = false, have_state = false;
     int filt_L = 256, filt_W = 40;
+    double filt_ms = -1.0;  // autotuned time of both filter sweeps at filt_L (-1: not tuned)
     hipGraphExec_t graph = nullptr;
     int stage_variant = -1;       // block-size variant of the stage kernel (-1: not tuned)
     double tune_ms[3] = {0, 0, 0};
@@ -391,28 +392,38 @@ int enqueue_step_t(ns_ctx *c, int iter, bool timed)
 
 int enqueue_step(ns_ctx *c, int iter, bool timed) { return enqueue_step_t(c, iter, timed); }
 
-// Autotuner: time each block-size variant of the stage kernel on the live
-// workload and keep the fastest.  It runs stage 1 (which does not read W)
-// from S0 into S1, so it leaves the state untouched when called between steps.
+// Autotuner: time each variant of the stage launch on the live workload, with
+// the same fork/join onto the side stream as a step, and keep the fastest.
+// It runs stage 2 (which reads and writes W, like stages 2-5) from S0 into
+// S1; between steps W is dead (stage 1 does not read it) and S1 is scratch,
+// so the state is untouched.
 template <typename S>
 int autotune_stage_t(ns_ctx *c)
 {
     const ns_params &P = c->P;
     const bool visc = !(P.flags & NS_FLAG_INVISCID);
     S *S0 = static_cast<S *>(c->S[0]), *S1 = static_cast<S *>(c->S[1]);
-    const StageArgs<S> A = stage_args<S>(c, 0, S0, S1, 0.0, c->pert);
+    const StageArgs<S> A = stage_args<S>(c, 1, S0, S1, 0.0, c->pert);
+    auto launch = [&](int v) -> hipError_t {
+        hipError_t e = hipEventRecord(c->fork, c->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->fork, 0);
+        if (e == hipSuccess) e = launch_stage(A, P.nk, visc, false, v, c->stream, c->side);
+        if (e == hipSuccess) e = hipEventRecord(c->join, c->side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->join, 0);
+        return e;
+    };
     int best = 0;
     double best_ms = 1e300;
     const int reps = 3;
     for (int v = 0; v < 3; ++v) {
-        hipError_t e = launch_stage(A, P.nk, visc, false, v, c->stream, nullptr);
+        hipError_t e = launch(v);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             c->tune_ms[v] = -1.0;
             continue;
         }
         HIPCHK(hipEventRecord(c->ev[0], c->stream));
-        for (int r = 0; r < reps; ++r) HIPCHK(launch_stage(A, P.nk, visc, false, v, c->stream, nullptr));
+        for (int r = 0; r < reps; ++r) HIPCHK(launch(v));
         HIPCHK(hipEventRecord(c->ev[1], c->stream));
         HIPCHK(hipEventSynchronize(c->ev[1]));
         float ms = 0.f;
@@ -424,6 +435,38 @@ int autotune_stage_t(ns_ctx *c)
         }
     }
     c->stage_variant = best;
+    // look-back chunk length of the filters (more chunks: more waves in
+    // flight; longer chunks: less warm-up work), timed on the scratch buffers
+    // S1 -> S2 -> S1 so the state is untouched.  Lines shorter than 272
+    // points keep L = 256: one chunk, the exact sequential sweep, so small
+    // grids stay bitwise equal to the reference order (a look-back start is
+    // exact to 1e-19 relative, which can still flip a rounding now and then).
+    S *S2 = static_cast<S *>(c->S[2]);
+    const int Ls[3] = {128, 256, 512};
+    int bestL = c->filt_L;
+    double bestL_ms = 1e300;
+    const int ntune = std::max(P.imax, P.jmax) < 272 ? 0 : 3;
+    for (int t = 0; t < ntune; ++t) {
+        c->filt_L = Ls[t];
+        auto sweep = [&]() -> hipError_t {
+            hipError_t e = launch_filter(filter_args<S>(c, true, S1, S2, false, c->pert), true, c->stream);
+            if (e == hipSuccess) e = launch_filter(filter_args<S>(c, false, S2, S1, true, c->pert), false, c->stream);
+            return e;
+        };
+        HIPCHK(sweep());
+        HIPCHK(hipEventRecord(c->ev[0], c->stream));
+        for (int r = 0; r < reps; ++r) HIPCHK(sweep());
+        HIPCHK(hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        if (ms / reps < bestL_ms) {
+            bestL_ms = ms / reps;
+            bestL = Ls[t];
+        }
+    }
+    c->filt_L = bestL;
+    if (ntune) c->filt_ms = bestL_ms;
     return NS_OK;
 }
 
@@ -1229,7 +1272,7 @@ int batch_enqueue_step(ns_batch *b)
     return b->pert ? batch_enqueue_step_t<float>(b) : batch_enqueue_step_t<double>(b);
 }
 
-// time the stage kernel's variants on stage 1 of the batch (W dead, S1 scratch)
+// time the stage kernel's variants on stage 2 of the batch (W dead between steps, S1 scratch)
 template <typename S>
 int batch_autotune_t(ns_batch *b)
 {
@@ -1238,7 +1281,7 @@ int batch_autotune_t(ns_batch *b)
     for (int v = 0; v < 3; ++v) {
         const StageArgs<S> *sa = static_cast<const StageArgs<S> *>(b->sargs);
         hipError_t e =
-            launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 0, b->n, b->nk, b->visc, v, b->stream, nullptr);
+            launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 1, b->n, b->nk, b->visc, v, b->stream, nullptr);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             b->tune_ms[v] = -1.0;
@@ -1246,7 +1289,7 @@ int batch_autotune_t(ns_batch *b)
         }
         HIPCHK(hipEventRecord(b->ev0, b->stream));
         for (int r = 0; r < 3; ++r)
-            HIPCHK(launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 0, b->n, b->nk, b->visc, v, b->stream,
+            HIPCHK(launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 1, b->n, b->nk, b->visc, v, b->stream,
                                       nullptr));
         HIPCHK(hipEventRecord(b->ev1, b->stream));
         HIPCHK(hipEventSynchronize(b->ev1));

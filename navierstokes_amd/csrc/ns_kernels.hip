@@ -594,18 +594,16 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
     return true;
 }
 
-// The stage: the interior region on stream s -- the marching kernel (FP64) or
-// fast tiles (FP32: the tile kernel is faster there, and keeps single cases
-// bitwise equal to the batched path, which uses tiles) -- and the general
+// The stage: the interior region on stream s -- the marching kernel (MARCH)
+// or fast tiles -- and the general
 // tile kernel on the rest of the grid in one launch (stream s2 when given, so
 // the two run concurrently; the caller orders s2 after s and joins it back).
 // The RHS-only mode runs the general tiles everywhere.
-template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, int RB>
+template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, int RB, bool MARCH>
 static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream_t s2)
 {
     using R = S;
     constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
-    constexpr bool MARCH = sizeof(S) == 8;
     MarchMap m{};
     int nchunks = 0;
     int fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
@@ -652,24 +650,28 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     return hipSuccess;
 }
 
-// variants (same arithmetic, bitwise identical results):
-//   FP64: 0 -> marching 4 rows/step, general tiles 256 threads;
-//         1 -> 4 rows, 512; 2 -> 8 rows/step, 512
-//   FP32: tiles of 256, 512 or 1024 threads
+// variants (same arithmetic):
+//   FP64: 0 -> marching 4 rows/step, edge tiles concurrent on s2;
+//         1 -> 8 rows/step, concurrent; 2 -> 8 rows/step, edge tiles after the
+//         interior on s (no concurrency)
+//   FP32: 0 -> interior tiles (512 threads); 1 -> marching 4 rows/step;
+//         2 -> marching 8 rows/step (edge tiles concurrent in all three)
+// FP64 results are bitwise identical across variants; in FP32 (contraction
+// allowed) the tile and marching kernels may round differently.
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
 static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s, hipStream_t s2)
 {
     if constexpr (sizeof(S) == 8) {
         switch (variant) {
-        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4>(A, s, s2);
-        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8>(A, s, s2);
-        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 256, 4>(A, s, s2);
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true>(A, s, s2);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true>(A, s, nullptr);
+        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
         }
     } else {
         switch (variant) {
-        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4>(A, s, s2);
-        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 1024, 4>(A, s, s2);
-        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 256, 4>(A, s, s2);
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true>(A, s, s2);
+        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, false>(A, s, s2);
         }
     }
 }
@@ -920,68 +922,71 @@ __device__ __forceinline__ bool chunk_bounds(int c, int L, int n, int &a, int &b
 
 // x direction: lanes are rows; LDS-staged 64x16 tiles keep global access
 // coalesced (lanes read/write 16-column row segments).
-constexpr int FX_T = 16;
-
+// Tiles are 64 rows x 16 columns (one 128-byte line per row segment in FP64;
+// 32-column FP32 tiles would fill lines too, but their 32 in-flight loads per
+// lane raise the register count enough to cost more occupancy than they save).
 template <typename R>
 struct XIO {
+    static constexpr int T = 16;
+    static constexpr int RP = 64 / T;  // rows per pass
     const R *in;
     R *out;
     int pitch, n, m, row0, a, b, lane;
-    R (*tin)[FX_T + 1];
-    R (*tout)[FX_T + 1];
+    R (*tin)[T + 1];
+    R (*tout)[T + 1];
     int tc;  // start of the tile in LDS
     int oc;  // start of the output tile being filled (-1 none)
-    R pre[FX_T];
+    R pre[T];
 
     __device__ __forceinline__ void fetch(int t)  // issue loads of tile t into registers
     {
-        const int c = lane & 15, rs = lane >> 4;
+        const int c = lane % T, rs = lane / T;
 #pragma unroll
-        for (int rr = 0; rr < 16; ++rr) {
-            const int gj = row0 + rr * 4 + rs, gi = t + c;
+        for (int rr = 0; rr < T; ++rr) {
+            const int gj = row0 + rr * RP + rs, gi = t + c;
             pre[rr] = (gj < m && gi < n && t >= 0) ? in[(size_t)gj * pitch + gi] : R(0);
         }
     }
     __device__ __forceinline__ void commit()
     {
-        const int c = lane & 15, rs = lane >> 4;
+        const int c = lane % T, rs = lane / T;
         __syncthreads();
 #pragma unroll
-        for (int rr = 0; rr < 16; ++rr) tin[rr * 4 + rs][c] = pre[rr];
+        for (int rr = 0; rr < T; ++rr) tin[rr * RP + rs][c] = pre[rr];
         __syncthreads();
     }
     __device__ __forceinline__ void start(int p0)
     {
-        tc = p0 & ~(FX_T - 1);
+        tc = p0 & ~(T - 1);
         fetch(tc);
         commit();
-        fetch(tc + FX_T);
+        fetch(tc + T);
         oc = -1;
     }
     __device__ __forceinline__ R load(int p)
     {
-        if (p >= tc + FX_T) {  // uniform: all lanes stream the same positions
+        if (p >= tc + T) {  // uniform: all lanes stream the same positions
             commit();
-            tc += FX_T;
-            fetch(tc + FX_T);
+            tc += T;
+            fetch(tc + T);
         }
         return tin[lane][p - tc];
     }
     __device__ __forceinline__ void flush()
     {
         if (oc < 0) return;
-        const int c = lane & 15, rs = lane >> 4;
+        const int c = lane % T, rs = lane / T;
         __syncthreads();
 #pragma unroll
-        for (int rr = 0; rr < 16; ++rr) {
-            const int gj = row0 + rr * 4 + rs, gi = oc + c;
-            if (gj < m && gi >= a && gi < b) out[(size_t)gj * pitch + gi] = tout[rr * 4 + rs][c];
+        for (int rr = 0; rr < T; ++rr) {
+            const int gj = row0 + rr * RP + rs, gi = oc + c;
+            if (gj < m && gi >= a && gi < b) out[(size_t)gj * pitch + gi] = tout[rr * RP + rs][c];
         }
         __syncthreads();
     }
     __device__ __forceinline__ void emit(int e, R v)
     {
-        const int t = e & ~(FX_T - 1);
+        const int t = e & ~(T - 1);
         if (t != oc) {
             flush();
             oc = t;
@@ -994,8 +999,9 @@ struct XIO {
 template <typename R>
 __device__ __forceinline__ void filter_x_tile(const FilterArgs<R> &F, int chunk, int rowblock, int k)
 {
-    __shared__ R tin[64][FX_T + 1];
-    __shared__ R tout[64][FX_T + 1];
+    constexpr int T = XIO<R>::T;
+    __shared__ R tin[64][T + 1];
+    __shared__ R tout[64][T + 1];
     int a, b;
     if (!chunk_bounds(chunk, F.L, F.n, a, b)) return;
     XIO<R> io;

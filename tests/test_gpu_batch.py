@@ -2,8 +2,10 @@
 
 The batch runs each kernel class as one launch over a tile table spanning all
 cases (and replays a captured graph per step), but each tile computes with its
-own case's arguments, so the results must be bitwise those of a single-case
-context on the same inputs, in both precisions."""
+own case's arguments, so in FP64 the results must be bitwise those of a
+single-case context on the same inputs; in the FP32 perturbation form they
+agree to FP32 rounding (the single context may run a different interior
+kernel)."""
 import numpy as np
 import pytest
 
@@ -17,6 +19,18 @@ FP64, FP32P = 0, 1
 
 def _uneven_pulses(seed):
     return config.batch_sweep_cases(5, seed=seed, sizes=(64, 96, 128, 160), D=12)
+
+
+def _same(got, ref, case, prec, what):
+    """FP64: bitwise.  FP32 perturbation form: a single context may run the
+    interior on the marching kernel (autotuned) while the batch runs tiles;
+    with contraction allowed the two round differently, so the bar is FP32
+    rounding of the perturbation (1e-6 of its largest value)."""
+    if prec == FP64:
+        assert np.array_equal(got, ref), f"{what}: max diff {np.abs(got - ref).max():.3e}"
+    else:
+        scale = np.abs(ref - case.Ub).max()
+        assert np.abs(got - ref).max() <= 1e-6 * scale, f"{what}: max diff {np.abs(got - ref).max():.3e}"
 
 
 def _singles(cases, prec, nsteps):
@@ -40,8 +54,7 @@ def test_batch_equals_single_euler(prec):
     b.run(0, 5)
     b.run(5, 7)
     for i, c in enumerate(cases):
-        got = b.get_state(i)
-        assert np.array_equal(got, ref[i]), f"case {i} ({c.name}): max diff {np.abs(got - ref[i]).max():.3e}"
+        _same(b.get_state(i), ref[i], c, prec, f"case {i} ({c.name})")
     b.close()
 
 
@@ -55,8 +68,8 @@ def test_batch_equals_single_viscous(nk):
         ref = _singles(cases, prec, 6)
         b = BatchSolver(cases, precision=prec)
         b.run(0, 6)
-        for i in range(len(cases)):
-            assert np.array_equal(b.get_state(i), ref[i]), f"prec {prec} case {i}"
+        for i, c in enumerate(cases):
+            _same(b.get_state(i), ref[i], c, prec, f"prec {prec} case {i}")
         b.close()
 
 
