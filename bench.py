@@ -19,6 +19,12 @@ observer arc, 500 steps with checkpoints every 50, FP64 and the FP32
 perturbation form; reports forward and backward times and checks the FP64
 gradient against central differences on 32 seeded directions.
 
+`--workload mixing`: 2-D compressible mixing layer, 2048 x 1024, uniform x,
+tanh-stretched y (spacing ratio 10) with metrics everywhere, u = 0.25
+tanh(2y/delta), Re_delta = 1000, Pr = 0.72, 60-point buffers, seeded v';
+FP64 and the FP32 perturbation form for the same steps (default 20000);
+reports both rates and the momentum-thickness growth-rate error of FP32.
+
 Prints one JSON line: Mpts*steps/s over the timed steps, the roofline of the
 dominant kernel (stage kernel, HBM bound) from HIP events on the library's
 stream, the whole-step roofline against the 704 B/pt/step compulsory
@@ -262,12 +268,77 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_mixing(args):
+    from navierstokes_amd import config
+    from navierstokes_amd.solver import Solver
+
+    nx, ny = (args.size, args.size // 2) if args.size else (2048, 1024)
+    nsteps = args.steps if args.steps != 50 else 20000
+    every = args.sample_every
+    case = config.mixing_layer_case(nx, ny, D=60)
+    y = config.mixing_layer_rows(case)
+    desc = (f"mixing layer {nx}x{ny}, tanh-stretched y (ratio 10, metrics everywhere), M=0.5, Re_delta=1000, "
+            f"Pr=0.72, D=60, {nsteps} steps, theta every {every}")
+    print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
+    res = {}
+    for prec, name in ((0, "fp64"), (1, "fp32pert")):
+        s = Solver(case, precision=prec)
+        s.autotune()
+        t_steps = 0.0
+        times, theta = [], []
+        n = 0
+        while True:
+            ub = s.row_average(1, case.D + 1, case.imax - case.D)
+            theta.append(config.momentum_thickness(ub, y, 0.25, -0.25))
+            times.append(n * case.dt)
+            if n >= nsteps:
+                break
+            k = min(every, nsteps - n)
+            t0 = time.perf_counter()
+            s.run(n, k)
+            t_steps += time.perf_counter() - t0
+            n += k
+            if n % (every * 10) == 0 or n == nsteps:
+                print(f"[bench] {name}: step {n}, theta {theta[-1]:.6f}", file=sys.stderr, flush=True)
+        s.check_finite()
+        s.close()
+        t = np.array(times)
+        th = np.array(theta)
+        half = len(t) // 2
+        rate = float(np.polyfit(t[half:], th[half:], 1)[0])
+        res[name] = {"ms_per_step": t_steps / nsteps * 1e3, "Mpts_steps_per_s": case.npoints * nsteps / t_steps / 1e6,
+                     "theta0": float(th[0]), "theta_end": float(th[-1]), "dtheta_dt": rate}
+    r64, r32 = res["fp64"]["dtheta_dt"], res["fp32pert"]["dtheta_dt"]
+    res["fp32pert"]["growth_rate_rel_err_vs_fp64"] = abs(r32 - r64) / abs(r64) if r64 != 0 else None
+    prec = args.precision or "fp32pert"
+    r = res[prec]
+    out = {
+        "metric": METRIC,
+        "value": round(r["Mpts_steps_per_s"], 3),
+        "unit": "Mpts·steps/s",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": 0,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32pert" else "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "grid": [nx, ny], "nk": 4, "precision": prec, "parallelism": "single-gpu"},
+        "roofline": None,
+        "mixing": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["ns", "pulse", "batch", "adjoint"], default="ns")
+    ap.add_argument("--workload", choices=["ns", "pulse", "batch", "adjoint", "mixing"], default="ns")
+    ap.add_argument("--sample-every", type=int, default=200, help="mixing: steps between theta samples")
     ap.add_argument("--fd-directions", type=int, default=32)
     ap.add_argument("--every", type=int, default=50, help="adjoint checkpoint interval")
     ap.add_argument("--ncases", type=int, default=256)
@@ -293,6 +364,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "mixing":
+        bench_mixing(args)
         return
     args.precision = args.precision or "fp64"
 

@@ -101,6 +101,12 @@ int32_t ns_get_state(ns_ctx *ctx, double *U);        /* device -> host AoS */
 int32_t ns_probe(ns_ctx *ctx, int32_t npts, const int32_t *ij, double *out);
 /* NS_ERR_NONFINITE if any state value is NaN/Inf */
 int32_t ns_check_finite(ns_ctx *ctx);
+/* out[j] = mean over columns i0..i1 (1-based, inclusive) of component k of
+   the state, j = 1..jmax (out has jmax values), reduced on the device.  For
+   the statistics of the reference's mixing-layer runs (the x-averaged
+   profiles behind its momentum thickness); no routine of the reference
+   computes it on this path. */
+int32_t ns_row_average(ns_ctx *ctx, int32_t k, int32_t i0, int32_t i1, double *out);
 
 /* time advance ------------------------------------------------------------- */
 /* one LDDRK6 step, rk.f90:12-75 (rk_euler(U,t,iter,nk)) */

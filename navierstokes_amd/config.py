@@ -350,6 +350,66 @@ def observer_arc(N: int, radius: float, theta0: float = 0.0, theta1: float = mat
     return np.asarray(pts, dtype=np.int32)
 
 
+def tanh_stretched_y(jmax: int, Ly: float, ratio: float):
+    """Rows y_j = (Ly/2) atanh(eta_j tanh b) / b on the uniform computational
+    coordinate eta in [-1, 1]; the spacing grows from the centre to the edges
+    by `ratio` (= cosh^2 b).  Returns (eta, y, dn, meshy) with meshy = d eta / dy,
+    the metric factor the reference multiplies y-derivatives by."""
+    b = math.acosh(math.sqrt(ratio))
+    eta = np.linspace(-1.0, 1.0, jmax)
+    dn = 2.0 / (jmax - 1)
+    tb = math.tanh(b)
+    y = 0.5 * Ly * np.arctanh(eta * tb) / b
+    dydeta = 0.5 * Ly * tb / (b * (1.0 - (eta * tb) ** 2))
+    return eta, y, dn, 1.0 / dydeta
+
+
+def mixing_layer_case(imax: int = 2048, jmax: int = 1024, D: int = 60, ratio: float = 10.0, delta: float = 1.0,
+                      M: float = 0.5, Re_delta: float = 1000.0, Pr: float = 0.72, amp: float = 1e-3,
+                      seed: int = 3, cells_per_delta: float = 20.0, nk: int = 4) -> Case:
+    """Temporal-style compressible mixing layer: uniform x (spacing equal to
+    the finest y spacing, delta / cells_per_delta), tanh-stretched y with
+    spacing ratio `ratio`, metric factors applied everywhere; base flow
+    u = 0.5 M tanh(2y/delta) with the Crocco-Busemann temperature
+    (mixing_layer_fields, init.f90:368-404); seeded perturbation
+    v' = amp N(0,1) exp(-(y/delta)^2); Re based on delta; buffers of D points
+    on all sides."""
+    rng = np.random.default_rng(seed)
+    b = math.acosh(math.sqrt(ratio))
+    dn = 2.0 / (jmax - 1)
+    dy_min = delta / cells_per_delta
+    Ly = 2.0 * dy_min * b / (math.tanh(b) * dn)
+    eta, y, dn, meshy = tanh_stretched_y(jmax, Ly, ratio)
+    dm = dy_min
+    half = 0.5 * (imax - 1) * dm
+    x = np.linspace(-half, half, imax)
+    g = grid_profiles(imax, jmax, -half, half, -1.0, 1.0, D, 0.01, 20.0)
+    rhob, psib, Ub = mixing_layer_fields(y, imax, nk, 0.5 * M, -0.5 * M, 1.0, 1.0, delta=delta)
+    U0 = Ub.copy()
+    U0[..., 2] += amp * rng.standard_normal((jmax, imax)) * np.exp(-(y / delta) ** 2)[:, None]
+    dt = 0.5 * dy_min / (1.0 + 0.5 * M)
+    return Case(name=f"mixing-layer-{imax}x{jmax}", imax=imax, jmax=jmax, nk=nk, D=D, dm=dm, dn=dn, dt=dt,
+                sigmaf=0.2, m=x, n=eta, meshx=np.ones(imax), meshy=meshy, sigmax=g["sigmax"], sigmay=g["sigmay"],
+                rhob=rhob, psib=psib, Ub=Ub, U0=U0, Re=Re_delta / delta, Pr=Pr, inviscid=False,
+                metric_everywhere=True, Tref=273.15)
+
+
+def mixing_layer_rows(case: Case, ratio: float = 10.0) -> np.ndarray:
+    """Physical y of the rows of a mixing_layer_case (its finest spacing is dm)."""
+    b = math.acosh(math.sqrt(ratio))
+    Ly = 2.0 * case.dm * b / (math.tanh(b) * case.dn)
+    return tanh_stretched_y(case.jmax, Ly, ratio)[1]
+
+
+def momentum_thickness(ubar: np.ndarray, y: np.ndarray, U1: float, U2: float, rho: np.ndarray | None = None) -> float:
+    """theta = int rho (U1 - u)(u - U2) / (rho_ref (U1 - U2)^2) dy over the
+    x-averaged profile (trapezoid rule on the physical rows); rho optional."""
+    f = (U1 - ubar) * (ubar - U2) / (U1 - U2) ** 2
+    if rho is not None:
+        f = f * rho
+    return float(np.trapezoid(f, y))
+
+
 def random_case(imax: int, jmax: int, nk: int = 4, D: int = 6, seed: int = 0, Re: float = 500.0,
                 amp: float = 1e-2, dt: float = 0.2, sigmaf: float = 0.2, inviscid: bool = False,
                 mesh_random: bool = True) -> Case:

@@ -33,6 +33,9 @@ hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles,
                               int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
 hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
                               int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
+template <typename S>
+hipError_t launch_row_mean(const S *U, int pitch, int jmax, int i0, int i1, const double *add, double *out,
+                           hipStream_t s);
 hipError_t launch_adj_stage(const AdjPointArgs<double> &P, const AdjGatherArgs<double> &G, hipStream_t s);
 hipError_t launch_adj_stage(const AdjPointArgs<float> &P, const AdjGatherArgs<float> &G, hipStream_t s);
 hipError_t launch_adj_filter(const AdjFilterArgs<double> &F, bool ydir, hipStream_t s);
@@ -917,6 +920,25 @@ int32_t ns_probe(ns_ctx *c, int32_t npts, const int32_t *ij, double *out)
     if (c->pert)
         for (int t = 0; t < npts; ++t)
             for (int k = 0; k < nk; ++k) out[t * nk + k] += c->ubh[(size_t)k * c->P.jmax + (ij[2 * t + 1] - 1)];
+    return NS_OK;
+}
+
+int32_t ns_row_average(ns_ctx *c, int32_t k, int32_t i0, int32_t i1, double *out)
+{
+    if (!c || !out) return fail(NS_ERR_PARAM, "null argument");
+    const ns_params &P = c->P;
+    if (k < 0 || k >= P.nk) return fail(NS_ERR_PARAM, "component %d outside 0..%d", k, P.nk - 1);
+    if (i0 < 1 || i1 > P.imax || i1 < i0) return fail(NS_ERR_PARAM, "column range %d..%d outside 1..%d", i0, i1, P.imax);
+    double *d = c->work;  // jmax values
+    const double *add = c->pert ? pert_rows(c).ub[k] : nullptr;
+    if (c->pert)
+        HIPCHK(launch_row_mean<float>(static_cast<const float *>(c->S[0]) + (size_t)k * c->plane, c->pitch, P.jmax,
+                                      i0 - 1, i1, add, d, c->stream));
+    else
+        HIPCHK(launch_row_mean<double>(static_cast<const double *>(c->S[0]) + (size_t)k * c->plane, c->pitch,
+                                       P.jmax, i0 - 1, i1, add, d, c->stream));
+    HIPCHK(hipMemcpyAsync(out, d, sizeof(double) * P.jmax, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return NS_OK;
 }
 

@@ -143,16 +143,19 @@ struct MarchMap {
 };
 
 // Output columns / rows of the marching region for strips of 58 and steps of
-// rb rows: every output's 3-point level-1 halo inside [max(D,4), n - max(D,4))
-// in both directions, rows a multiple of rb.  False when it is too small.
-__host__ __device__ inline bool march_region(int fast_ok, int D, int imax, int jmax, int rb, MarchMap &m)
+// rb rows, rows a multiple of rb.  fast: every output's 3-point level-1 halo
+// inside [max(D,4), n - max(D,4)) in both directions (no buffer, metric or
+// one-sided stencil there).  General: outputs in [9, n - 9), so the U window
+// (6-point halo) stays inside the grid and every derivative in it uses the
+// interior stencil; buffers and metrics are then handled point by point.
+// False when the region is too small.
+__host__ __device__ inline bool march_region(bool fast, int D, int imax, int jmax, int rb, MarchMap &m)
 {
-    if (!fast_ok) return false;
-    const int Dm = D > 4 ? D : 4;
-    m.xa = Dm + 3;
-    m.nstrips = (imax - Dm - 3 - m.xa) / 58;
-    m.ya = Dm + 3;
-    const int rows = jmax - Dm - 3 - m.ya;
+    const int lo = fast ? (D > 4 ? D : 4) + 3 : 9;
+    m.xa = lo;
+    m.nstrips = (imax - lo - m.xa) / 58;
+    m.ya = lo;
+    const int rows = jmax - lo - m.ya;
     m.yb = m.ya + (rows > 0 ? rows / rb * rb : 0);
     m.H = 0;
     return m.nstrips >= 1 && m.yb - m.ya >= 4 * rb;

@@ -377,7 +377,9 @@ __global__ __launch_bounds__(NT) void stage_kernel_batch(const StageArgs<S> *arg
 }
 
 // ---------------------------------------------------------------------------
-// Marching version (interior region only).  A block owns a strip of MTX = 58
+// Marching version (interior region; FAST: the fast zone, else the general
+// region of march_region with buffers and metrics point by point).  A block
+// owns a strip of MTX = 58
 // output columns, so its level-1 rows are exactly 64 wide (one wave per row)
 // and its U window rows 70 wide, and marches up a chunk of rows RB at a time
 // (RB waves).  Window rows and level-1 rows live in LDS rings: every row is
@@ -409,7 +411,7 @@ static size_t march_smem(int NK, bool visc, int RB)
     return sizeof(R) * ((size_t)(NK + (visc ? 1 : 0)) * WR * MWX + (size_t)NL * LR * MLX);
 }
 
-template <typename S, typename R, int NK, bool VISC, bool PERT, int RB>
+template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, bool FAST>
 __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, MarchMap m)
 {
     constexpr int NT = RB * 64, WR = MarchRing<RB>::WR, LR = MarchRing<RB>::LR;
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
     auto level1_row = [&](int r) {  // wave-uniform row r, column lx0 + lane
         const int gx = lx0 + lane;
         R out[NL > 0 ? NL : 1];
-        level1_point<S, R, NK, VISC, PERT, true>(A, tc, gx, r, ux_at(r), uy_at(gx), tx_at(r), ty_at(gx),
+        level1_point<S, R, NK, VISC, PERT, FAST>(A, tc, gx, r, ux_at(r), uy_at(gx), tx_at(r), ty_at(gx),
                                                   VISC ? tx_at(r)(gx) : R(0), out);
         const int s = (r - ly0) % LR;
 #pragma unroll
@@ -543,8 +545,8 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
             };
             auto lv = [=](int l) { return sL[(l * LR + ls) * MLX + (gx - lx0)]; };
             R qc[NK], uc[NK], f[NK];
-            rhs_point<S, R, NK, VISC, PERT, true>(A, gx, gy, ux_at(gy), uy_at(gx), lxa, lya, lv, qc, uc, f);
-            stage_store<S, R, NK, false, PERT, true>(A, gx, gy, qc, uc, f, wpre);
+            rhs_point<S, R, NK, VISC, PERT, FAST>(A, gx, gy, ux_at(gy), uy_at(gx), lxa, lya, lv, qc, uc, f);
+            stage_store<S, R, NK, false, PERT, FAST>(A, gx, gy, qc, uc, f, wpre);
         }
     }
 }
@@ -561,10 +563,11 @@ static size_t stage_smem(int NK, bool visc, int TY)
 // every point's 3-point level-1 halo inside the fast zone (interior, outside
 // the buffer/metric layers), rows a multiple of RB; chunk height H from a
 // simple occupancy model (blocks per wave of the GPU x rows per block).
-template <typename S, int NK, bool VISC, bool PERT, int RB>
+template <typename S, int NK, bool VISC, bool PERT, int RB, bool FAST>
 static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
 {
-    if (!march_region(A.fast_ok, A.D, A.imax, A.jmax, RB, m)) return false;
+    if (FAST && !A.fast_ok) return false;
+    if (!march_region(FAST, A.D, A.imax, A.jmax, RB, m)) return false;
     const int rows = m.yb - m.ya;
     static int ncu = 0, perblk = 0;
     if (!ncu) {
@@ -572,7 +575,7 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (ncu <= 0) ncu = 256;
-        auto kern = stage_march_kernel<S, S, NK, VISC, PERT, RB>;
+        auto kern = stage_march_kernel<S, S, NK, VISC, PERT, RB, FAST>;
         const size_t smem = march_smem<S>(NK, VISC, RB);
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perblk, kern, RB * 64, smem) != hipSuccess || perblk < 1)
@@ -599,7 +602,7 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
 // tile kernel on the rest of the grid in one launch (stream s2 when given, so
 // the two run concurrently; the caller orders s2 after s and joins it back).
 // The RHS-only mode runs the general tiles everywhere.
-template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, int RB, bool MARCH>
+template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, int RB, bool MARCH, bool GENERAL = false>
 static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream_t s2)
 {
     using R = S;
@@ -607,10 +610,13 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     MarchMap m{};
     int nchunks = 0;
     int fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
-    bool inner = false;
+    bool inner = false, fastm = false;
     RectSet rs;
     if constexpr (MARCH) {
-        inner = !RHS_ONLY && march_plan<S, NK, VISC, PERT, RB>(A, m, nchunks);
+        // the fast zone when it exists (unless GENERAL), else the general region
+        fastm = !GENERAL && A.fast_ok;
+        inner = !RHS_ONLY && (fastm ? march_plan<S, NK, VISC, PERT, RB, true>(A, m, nchunks)
+                                    : march_plan<S, NK, VISC, PERT, RB, false>(A, m, nchunks));
         rs = inner ? RectSet::around(A.imax, A.jmax, TY, m.xa, m.xa + m.nstrips * MTX, m.ya, m.yb)
                    : RectSet::around(A.imax, A.jmax, TY, 0, 0, 0, 0);
     } else {
@@ -632,7 +638,8 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     }
     if (!inner) return hipSuccess;
     if constexpr (!RHS_ONLY && MARCH) {
-        auto kern = stage_march_kernel<S, R, NK, VISC, PERT, RB>;
+        auto kern = fastm ? stage_march_kernel<S, R, NK, VISC, PERT, RB, true>
+                          : stage_march_kernel<S, R, NK, VISC, PERT, RB, false>;
         const size_t msm = march_smem<R>(NK, VISC, RB);
         if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)msm)) !=
             hipSuccess)
@@ -651,26 +658,30 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
 }
 
 // variants (same arithmetic):
-//   FP64: 0 -> marching 4 rows/step, edge tiles concurrent on s2;
-//         1 -> 8 rows/step, concurrent; 2 -> 8 rows/step, edge tiles after the
-//         interior on s (no concurrency)
-//   FP32: 0 -> interior tiles (512 threads); 1 -> marching 4 rows/step;
-//         2 -> marching 8 rows/step (edge tiles concurrent in all three)
-// FP64 results are bitwise identical across variants; in FP32 (contraction
-// allowed) the tile and marching kernels may round differently.
+//   FP64: 0 -> marching 4 rows/step over the fast zone (the general region
+//         when there is none), edge tiles concurrent on s2;
+//         1 -> marching 4 rows/step over the general region [9, n-9)^2
+//         (buffers and metrics point by point), thin edge band concurrent;
+//         2 -> as 0 with 8 rows/step
+//   FP32: 0 -> interior tiles (512 threads) over the fast zone, general tiles
+//         elsewhere; 1 -> marching 4 rows/step over the general region;
+//         2 -> as 0 of FP64
+// FP64 results are bitwise identical across variants (the general per-point
+// code at unit metric and spacing performs the same operations); in FP32
+// (contraction allowed) the tile and marching kernels may round differently.
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
 static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s, hipStream_t s2)
 {
     if constexpr (sizeof(S) == 8) {
         switch (variant) {
-        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true>(A, s, s2);
-        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true>(A, s, nullptr);
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true, true>(A, s, s2);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true>(A, s, s2);
         default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
         }
     } else {
         switch (variant) {
-        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
-        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true>(A, s, s2);
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true, true>(A, s, s2);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
         default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, false>(A, s, s2);
         }
     }
@@ -1384,6 +1395,42 @@ template hipError_t launch_probe<double>(const double *const *, int, int, int, c
 #endif
 #if defined(NS_KERNELS_F32)
 template hipError_t launch_probe<float>(const float *const *, int, int, int, const int *, double *, hipStream_t);
+#endif
+
+// Row averages of one plane: out[j] = mean over i in [i0, i1) of U(i, j)
+// (+ add[j] when given: the base row of the perturbation form).  One block
+// per row, FP64 sums in a fixed order (deterministic).
+template <typename S>
+__global__ __launch_bounds__(256) void row_mean_kernel(const S *U, int pitch, int i0, int i1, const double *add,
+                                                       double *out)
+{
+    __shared__ double part[256];
+    const int j = blockIdx.x;
+    const S *row = U + (size_t)j * pitch;
+    double s = 0.0;
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) s += double(row[i]);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = part[0] / double(i1 - i0) + (add ? add[j] : 0.0);
+}
+
+template <typename S>
+hipError_t launch_row_mean(const S *U, int pitch, int jmax, int i0, int i1, const double *add, double *out,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(row_mean_kernel<S>, dim3(jmax), dim3(256), 0, s, U, pitch, i0, i1, add, out);
+    return hipGetLastError();
+}
+#if defined(NS_KERNELS_F64)
+template hipError_t launch_row_mean<double>(const double *, int, int, int, int, const double *, double *,
+                                            hipStream_t);
+#endif
+#if defined(NS_KERNELS_F32)
+template hipError_t launch_row_mean<float>(const float *, int, int, int, int, const double *, double *, hipStream_t);
 #endif
 
 // non-finite scan: flag = 1 if any value is NaN/Inf

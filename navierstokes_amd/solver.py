@@ -123,6 +123,13 @@ class Solver:
                                      _lib.dptr(out)))
         return out
 
+    def row_average(self, k: int, i0: int = 1, i1: int | None = None) -> np.ndarray:
+        """Mean of component k over columns i0..i1 (1-based, inclusive) per row."""
+        i1 = self.case.imax if i1 is None else i1
+        out = np.empty(self.case.jmax)
+        _lib.check(self.lib.ns_row_average(self.ctx, int(k), int(i0), int(i1), _lib.dptr(out)))
+        return out
+
     def check_finite(self):
         _lib.check(self.lib.ns_check_finite(self.ctx))
 

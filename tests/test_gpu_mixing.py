@@ -1,0 +1,52 @@
+"""Mixing layer on the tanh-stretched grid (metrics everywhere): the device
+row average, and the FP32 perturbation form against FP64 through the
+momentum thickness the mixing-layer runs are judged by."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from navierstokes_amd import config
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(case, prec):
+    from navierstokes_amd.solver import Solver
+    print(f"[case] {case.name} precision={prec}", flush=True)
+    return Solver(case, precision=prec)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_row_average(prec):
+    case = config.mixing_layer_case(160, 96, D=10)
+    s = _solver(case, prec)
+    s.run(0, 3)
+    U = s.get_state()
+    for k in range(4):
+        got = s.row_average(k, 11, 150)
+        ref = U[:, 10:150, k].mean(axis=1)
+        assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max() + 1e-300
+    s.close()
+
+
+def test_fp32_momentum_thickness():
+    case = config.mixing_layer_case(256, 160, D=12)
+    y = config.mixing_layer_rows(case)
+    D = case.D
+    s64, s32 = _solver(case, 0), _solver(case, 1)
+    th = {0: [], 1: []}
+    for n in range(6):
+        for p, s in ((0, s64), (1, s32)):
+            if n:
+                s.run((n - 1) * 100, 100)
+            ub = s.row_average(1, D + 1, case.imax - D)
+            th[p].append(config.momentum_thickness(ub, y, 0.25, -0.25))
+    t64, t32 = np.array(th[0]), np.array(th[1])
+    print("theta fp64", t64, "fp32-pert", t32, flush=True)
+    assert np.abs(t32 - t64).max() < 1e-5 * t64.max()
+    a, b_ = s64.get_state(), s32.get_state()
+    err = rel_l2(b_[..., 2], a[..., 2])
+    print(f"v rel L2 after 500 steps: {err:.3e}", flush=True)
+    assert err < 1e-4
+    s64.close()
+    s32.close()
