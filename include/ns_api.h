@@ -101,6 +101,11 @@ int32_t ns_get_state(ns_ctx *ctx, double *U);        /* device -> host AoS */
 int32_t ns_probe(ns_ctx *ctx, int32_t npts, const int32_t *ij, double *out);
 /* NS_ERR_NONFINITE if any state value is NaN/Inf */
 int32_t ns_check_finite(ns_ctx *ctx);
+/* outt.f90:49-55: the vorticity column of the snapshot files,
+   omega = dv/dm - du/dn with the DRP operators of diffall.f90 (no metric
+   factors), computed on the device from the current state.  omega: host
+   array of jmax x imax values, row-major (omega[j*imax + i]). */
+int32_t ns_vorticity(ns_ctx *ctx, double *omega);
 /* out[j] = mean over columns i0..i1 (1-based, inclusive) of component k of
    the state, j = 1..jmax (out has jmax values), reduced on the device.  For
    the statistics of the reference's mixing-layer runs (the x-averaged

@@ -32,10 +32,12 @@ struct AdjGatherArgs {
 
 template <typename S>
 struct AdjFilterArgs {
-    S *lam[NKMAX];  // transformed in place
+    const S *in[NKMAX];
+    S *out[NKMAX];  // a different buffer: chunks read each other's inputs
     int n;          // points along the filtered direction
     int m;          // lines
     int pitch, nk;
+    int L, W;       // chunk length, warm-up length (as the forward filter)
     double sigma;
 };
 

@@ -64,7 +64,7 @@ __global__ void adj_edge_kernel(AdjEdgeArgs<S> E, int phase)
 }
 
 // ---------------------------------------------------------------------------
-// F^T on one line of n points (0-based positions), in place.  The forward
+// F^T on one line of n points (0-based positions).  The forward
 // (filtering.f90:36-94) applies, in order,
 //   L: q = 1..6        u(q) -= sigma * sum_{s=-1..5} dfb(s) u(q+s)
 //   I: q = 6..n-7      u(q) -= sigma * sum_{s=-6..6} d(s)   u(q+s)
@@ -72,8 +72,17 @@ __global__ void adj_edge_kernel(AdjEdgeArgs<S> E, int phase)
 // each a rank-one update x <- (I - sigma e_q a^T) x.  Its transpose
 // lam <- (I - sigma a e_q^T) lam is "t = sigma lam(q); lam(c) -= a_c t", and
 // the transposed sweep runs the updates in reverse order: R ascending,
-// I descending, L descending.  One thread per line, a 13-value window in
-// registers (rotating, unrolled by 13) holding positions q-6..q+6.
+// I descending, L descending.
+//
+// Chunked like the forward filter: one lane computes the outputs [a, b) of
+// a line.  After op q (descending) position q+6 is final (later ops touch
+// q+5 at most), so the lane sweeps from q0 down to a-6 and emits q+6 when it
+// lies in [a, b).  Near the right end q0 = n-7 after R^T (exact); elsewhere
+// q0 = b+6+W starts from the unmodified input: the ops above q0 are missing,
+// and their effect on t_q decays like the forward recurrence's (|h_n| below
+// 1e-19 in total past W).  The chunk holding position 0 ends with L^T.
+// A 13-value window in registers (rotating, unrolled by 13) holds
+// positions q-6..q+6.
 // ---------------------------------------------------------------------------
 template <typename S>
 struct LineRef {
@@ -83,23 +92,26 @@ struct LineRef {
 };
 
 template <typename S, bool YDIR>
-__host__ __device__ __forceinline__ void adj_filter_line(const LineRef<S> &L, int n, S sigma)
+__host__ __device__ __forceinline__ void adj_filter_line(const LineRef<const S> &In, const LineRef<S> &Out, int n,
+                                                         S sigma, int a, int b, int W)
 {
+    const bool exact_right = b + 6 + W >= n - 7;
+    int q = exact_right ? n - 7 : b + 6 + W;
+    const int qs = a - 6 > 6 ? a - 6 : 6;
     S w[13];
-    // window = positions n-13 .. n-1 (slot k = position n-13+k)
+    // window = positions q-6 .. q+6 (slot k = position q-6+k)
 #pragma unroll
-    for (int k = 0; k < 13; ++k) w[k] = L(n - 13 + k);
-    // R^T: q = n-7 .. n-2 ascending; slot of position q-s is q-s-(n-13)
+    for (int k = 0; k < 13; ++k) w[k] = In(q - 6 + k);
+    if (exact_right) {
+        // R^T: ops n-7 .. n-2 ascending; slot of position p is p-(n-13)
 #pragma unroll
-    for (int qq = 6; qq <= 11; ++qq) {  // qq = q - (n-13)
-        const S t = sigma * w[qq];
+        for (int qq = 6; qq <= 11; ++qq) {
+            const S t = sigma * w[qq];
 #pragma unroll
-        for (int s = -1; s <= 5; ++s) w[qq - s] = w[qq - s] - S(filt_dfb<YDIR>(s + 1)) * t;
+            for (int s = -1; s <= 5; ++s) w[qq - s] = w[qq - s] - S(filt_dfb<YDIR>(s + 1)) * t;
+        }
     }
-    // I^T: q = n-7 down to 6.  At step u (q = n-7-u) position q+s sits in slot
-    // (6 + s - u) mod 13; after the update position q+6 is final (later steps
-    // touch q-1+6 at most) and its slot takes position q-7.
-    int q = n - 7;
+    // I^T from q down to qs.  At step u position q+s sits in slot (6+s-u) mod 13.
     int rot = 0;
     for (;;) {
         bool done = false;
@@ -112,56 +124,62 @@ __host__ __device__ __forceinline__ void adj_filter_line(const LineRef<S> &L, in
                     const int sl = (6 + s - u + 26) % 13;
                     w[sl] = w[sl] - S(filt_d(s + 6)) * t;
                 }
-                if (q == 6) {
+                const int sl = (12 - u + 13) % 13;  // position q+6, final now
+                if (q + 6 >= a && q + 6 < b) Out(q + 6) = w[sl];
+                if (q == qs) {
                     rot = u;
                     done = true;
                 } else {
-                    const int sl = (12 - u + 13) % 13;
-                    L(q + 6) = w[sl];
-                    w[sl] = L(q - 7);
+                    w[sl] = In(q - 7);
                     --q;
                 }
             }
         }
         if (done) break;
     }
-    // window now holds positions 0..12: position p in slot (p - u) mod 13 with
-    // u = rot (q = 6 = n-7-u' for the last u'); position p = q - 6 + k = k
-    S v[13];
+    if (qs == 6) {
+        // window = positions 0..12, position p in slot (p - rot) mod 13; L^T on 0..11
+        S v[13];
 #pragma unroll
-    for (int k = 0; k < 13; ++k) {
-        S x = w[0];
+        for (int k = 0; k < 13; ++k) {
+            S x = w[0];
 #pragma unroll
-        for (int r = 0; r < 13; ++r)
-            if (r == rot) x = w[(k - r + 13) % 13];
-        v[k] = x;
+            for (int r = 0; r < 13; ++r)
+                if (r == rot) x = w[(k - r + 13) % 13];
+            v[k] = x;
+        }
+#pragma unroll
+        for (int qq = 6; qq >= 1; --qq) {
+            const S t = sigma * v[qq];
+#pragma unroll
+            for (int s = -1; s <= 5; ++s) v[qq + s] = v[qq + s] - S(filt_dfb<YDIR>(s + 1)) * t;
+        }
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            if (k >= a && k < b) Out(k) = v[k];
     }
-    // L^T: q = 6 down to 1
-#pragma unroll
-    for (int qq = 6; qq >= 1; --qq) {
-        const S t = sigma * v[qq];
-#pragma unroll
-        for (int s = -1; s <= 5; ++s) v[qq + s] = v[qq + s] - S(filt_dfb<YDIR>(s + 1)) * t;
-    }
-#pragma unroll
-    for (int k = 0; k < 13; ++k) L(k) = v[k];
 }
 
 template <typename S, bool YDIR>
 __global__ __launch_bounds__(64) void adj_filter_kernel(AdjFilterArgs<S> F)
 {
     const int line = blockIdx.x * 64 + threadIdx.x;
-    const int k = blockIdx.y;
+    const int k = blockIdx.z;
     if (line >= F.m || k >= F.nk) return;
-    LineRef<S> L;
+    int a, b;
+    if (!chunk_bounds(blockIdx.y, F.L, F.n, a, b)) return;
+    LineRef<const S> In;
+    LineRef<S> Out;
     if (YDIR) {  // line = column i, positions along j
-        L.p = F.lam[k] + line;
-        L.st = (size_t)F.pitch;
+        In.p = F.in[k] + line;
+        Out.p = F.out[k] + line;
+        In.st = Out.st = (size_t)F.pitch;
     } else {  // line = row j, positions along i
-        L.p = F.lam[k] + (size_t)line * F.pitch;
-        L.st = 1;
+        In.p = F.in[k] + (size_t)line * F.pitch;
+        Out.p = F.out[k] + (size_t)line * F.pitch;
+        In.st = Out.st = 1;
     }
-    adj_filter_line<S, YDIR>(L, F.n, S(F.sigma));
+    adj_filter_line<S, YDIR>(In, Out, F.n, S(F.sigma), a, b, F.W);
 }
 
 // ---------------------------------------------------------------------------
@@ -394,7 +412,7 @@ static hipError_t launch_adj_stage_t(const AdjPointArgs<S> &P, const AdjGatherAr
 template <typename S>
 static hipError_t launch_adj_filter_t(const AdjFilterArgs<S> &F, bool ydir, hipStream_t s)
 {
-    const dim3 grid((F.m + 63) / 64, F.nk);
+    const dim3 grid((F.m + 63) / 64, (F.n + F.L - 1) / F.L, F.nk);
     if (ydir) hipLaunchKernelGGL((adj_filter_kernel<S, true>), grid, dim3(64), 0, s, F);
     else hipLaunchKernelGGL((adj_filter_kernel<S, false>), grid, dim3(64), 0, s, F);
     return hipGetLastError();

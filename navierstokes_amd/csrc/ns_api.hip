@@ -34,6 +34,9 @@ hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles,
 hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
                               int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
 template <typename S>
+hipError_t launch_vorticity(const S *u, const S *v, const double *ub1, int imax, int jmax, int pitch, double inv_dm,
+                            double inv_dn, double *out, hipStream_t s);
+template <typename S>
 hipError_t launch_row_mean(const S *U, int pitch, int jmax, int i0, int i1, const double *add, double *out,
                            hipStream_t s);
 hipError_t launch_adj_stage(const AdjPointArgs<double> &P, const AdjGatherArgs<double> &G, hipStream_t s);
@@ -923,6 +926,27 @@ int32_t ns_probe(ns_ctx *c, int32_t npts, const int32_t *ij, double *out)
     return NS_OK;
 }
 
+int32_t ns_vorticity(ns_ctx *c, double *omega)
+{
+    if (!c || !omega) return fail(NS_ERR_PARAM, "null argument");
+    const ns_params &P = c->P;
+    if (P.nk < 3) return fail(NS_ERR_PARAM, "vorticity needs u and v (nk >= 3)");
+    double *d = c->work;  // imax*jmax values
+    const size_t n = (size_t)P.imax * P.jmax;
+    if (c->pert) {
+        const float *q = static_cast<const float *>(c->S[0]);
+        HIPCHK(launch_vorticity<float>(q + c->plane, q + 2 * c->plane, pert_rows(c).ub[1], P.imax, P.jmax, c->pitch,
+                                       1.0 / P.dm, 1.0 / P.dn, d, c->stream));
+    } else {
+        const double *U = static_cast<const double *>(c->S[0]);
+        HIPCHK(launch_vorticity<double>(U + c->plane, U + 2 * c->plane, nullptr, P.imax, P.jmax, c->pitch,
+                                        1.0 / P.dm, 1.0 / P.dn, d, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(omega, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
 int32_t ns_row_average(ns_ctx *c, int32_t k, int32_t i0, int32_t i1, double *out)
 {
     if (!c || !out) return fail(NS_ERR_PARAM, "null argument");
@@ -1531,9 +1555,9 @@ void set_planes(S **dst, S *buf, const ns_ctx *c)
     for (int k = 0; k < NKMAX; ++k) dst[k] = k < c->P.nk ? buf + (size_t)k * c->plane : nullptr;
 }
 
-// ccpml^T, Fy^T, Fx^T on lam (in place)
+// ccpml^T, Fy^T, Fx^T on lam (result in lam; tmp: a state-sized scratch)
 template <typename S>
-int adj_linear_back(ns_ctx *c, S *lam, bool edge, bool fy, bool fx)
+int adj_linear_back(ns_ctx *c, S *lam, S *tmp, bool edge, bool fy, bool fx)
 {
     const ns_params &P = c->P;
     if (edge) {
@@ -1546,19 +1570,27 @@ int adj_linear_back(ns_ctx *c, S *lam, bool edge, bool fy, bool fx)
         HIPCHK(launch_adj_edge(E, c->stream));
     }
     AdjFilterArgs<S> F{};
-    set_planes(F.lam, lam, c);
     F.pitch = c->pitch;
     F.nk = P.nk;
     F.sigma = P.sigmaf;
-    if (fy) {
-        F.n = P.jmax;
-        F.m = P.imax;
-        HIPCHK(launch_adj_filter(F, true, c->stream));
-    }
-    if (fx) {
-        F.n = P.imax;
-        F.m = P.jmax;
-        HIPCHK(launch_adj_filter(F, false, c->stream));
+    F.L = 64;  // short chunks: the transposed sweeps are latency-bound
+    F.W = c->filt_W;
+    auto run = [&](S *in, S *out, bool ydir) -> int {
+        for (int k = 0; k < NKMAX; ++k) {
+            F.in[k] = k < P.nk ? in + (size_t)k * c->plane : nullptr;
+            F.out[k] = k < P.nk ? out + (size_t)k * c->plane : nullptr;
+        }
+        F.n = ydir ? P.jmax : P.imax;
+        F.m = ydir ? P.imax : P.jmax;
+        HIPCHK(launch_adj_filter(F, ydir, c->stream));
+        return NS_OK;
+    };
+    int rc;
+    if (fy && fx) {
+        if ((rc = run(lam, tmp, true)) || (rc = run(tmp, lam, false))) return rc;
+    } else if (fy || fx) {
+        if ((rc = run(lam, tmp, fy))) return rc;
+        HIPCHK(hipMemcpyAsync(lam, tmp, (size_t)P.nk * c->plane * sizeof(S), hipMemcpyDeviceToDevice, c->stream));
     }
     return NS_OK;
 }
@@ -1569,7 +1601,7 @@ int adj_stage_back(ns_ctx *c, int irk, S *Us, S *lam, S *lamW, S *gx, S *gy, boo
 {
     const ns_params &P = c->P;
     if (!rhs_only) {
-        int rc = adj_linear_back<S>(c, lam, true, true, true);
+        int rc = adj_linear_back<S>(c, lam, gx, true, true, true);  // gx is free until the point kernel
         if (rc) return rc;
     }
     AdjPointArgs<S> A{};
@@ -1892,10 +1924,10 @@ static int vjp_linear(ns_ctx *c, double *lam, bool edge, bool fy, bool fx)
 {
     if (!c || !lam) return fail(NS_ERR_PARAM, "null argument");
     if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
-    double *d = c->work;
+    double *d = c->work, *tmp = c->work + 8 * c->plane;
     int rc;
     if ((rc = upload_aos(c, d, lam, c->P.nk))) return rc;
-    if ((rc = adj_linear_back<double>(c, d, edge, fy, fx))) return rc;
+    if ((rc = adj_linear_back<double>(c, d, tmp, edge, fy, fx))) return rc;
     return download_aos(c, d, lam, c->P.nk);
 }
 

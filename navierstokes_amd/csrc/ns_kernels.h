@@ -81,6 +81,17 @@ struct EdgeArgs {
     double gamma;
 };
 
+// Filter chunk bounds: chunks of length L; a chunk that would end within 16
+// points of the line end absorbs the rest; a chunk starting there is empty.
+__host__ __device__ inline bool chunk_bounds(int c, int L, int n, int &a, int &b)
+{
+    a = c * L;
+    b = a + L < n ? a + L : n;
+    if (a > 0 && a > n - 16) return false;
+    if (b > n - 16) b = n;
+    return true;
+}
+
 // Range of tiles that may take the fast path (unit grid, no metric or buffer
 // work on the tile or its halo): [b0, b1) along
 // one direction of n points with tile size T.  Empty when fast_ok is 0.

@@ -920,17 +920,6 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
     io.finish();
 }
 
-// chunk bounds: chunks of length L; a chunk that would end within 16 points
-// of the line end absorbs the rest; a chunk starting there is empty.
-__device__ __forceinline__ bool chunk_bounds(int c, int L, int n, int &a, int &b)
-{
-    a = c * L;
-    b = min(a + L, n);
-    if (a > 0 && a > n - 16) return false;
-    if (b > n - 16) b = n;
-    return true;
-}
-
 // x direction: lanes are rows; LDS-staged 64x16 tiles keep global access
 // coalesced (lanes read/write 16-column row segments).
 // Tiles are 64 rows x 16 columns (one 128-byte line per row segment in FP64;
@@ -1395,6 +1384,42 @@ template hipError_t launch_probe<double>(const double *const *, int, int, int, c
 #endif
 #if defined(NS_KERNELS_F32)
 template hipError_t launch_probe<float>(const float *const *, int, int, int, const int *, double *, hipStream_t);
+#endif
+
+// Vorticity of the output path (outt.f90:49-55): omega = dv/dm - du/dn with
+// the DRP operators of diffall.f90 (no metric factors), from the state; in
+// the perturbation form U = Ub(j) + q, and d Ub/dn is taken on the FP64 base
+// row with the same operator.  out: jmax x imax, row-major.
+template <typename S>
+__global__ __launch_bounds__(256) void vorticity_kernel(const S *u, const S *v, const double *ub1, int imax, int jmax,
+                                                        int pitch, double inv_dm, double inv_dn, double *out)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= imax || j >= jmax) return;
+    const S *vrow = v + (size_t)j * pitch;
+    const S *ucol = u + i;
+    const double dvdm = drp_deriv<double>([vrow](int q) { return double(vrow[q]); }, i, imax, inv_dm);
+    double dudn = drp_deriv<double>([ucol, pitch](int q) { return double(ucol[(size_t)q * pitch]); }, j, jmax, inv_dn);
+    if (ub1) dudn = dudn + drp_deriv<double>([ub1](int q) { return ub1[q]; }, j, jmax, inv_dn);
+    out[(size_t)j * imax + i] = dvdm - dudn;
+}
+
+template <typename S>
+hipError_t launch_vorticity(const S *u, const S *v, const double *ub1, int imax, int jmax, int pitch, double inv_dm,
+                            double inv_dn, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(vorticity_kernel<S>, dim3((imax + 63) / 64, (jmax + 3) / 4), dim3(256), 0, s, u, v, ub1, imax,
+                       jmax, pitch, inv_dm, inv_dn, out);
+    return hipGetLastError();
+}
+#if defined(NS_KERNELS_F64)
+template hipError_t launch_vorticity<double>(const double *, const double *, const double *, int, int, int, double,
+                                             double, double *, hipStream_t);
+#endif
+#if defined(NS_KERNELS_F32)
+template hipError_t launch_vorticity<float>(const float *, const float *, const double *, int, int, int, double,
+                                            double, double *, hipStream_t);
 #endif
 
 // Row averages of one plane: out[j] = mean over i in [i0, i1) of U(i, j)

@@ -123,6 +123,12 @@ class Solver:
                                      _lib.dptr(out)))
         return out
 
+    def vorticity(self) -> np.ndarray:
+        """dv/dm - du/dn of the current state, shape (jmax, imax) (outt.f90:49-55)."""
+        out = np.empty((self.case.jmax, self.case.imax))
+        _lib.check(self.lib.ns_vorticity(self.ctx, _lib.dptr(out)))
+        return out
+
     def row_average(self, k: int, i0: int = 1, i1: int | None = None) -> np.ndarray:
         """Mean of component k over columns i0..i1 (1-based, inclusive) per row."""
         i1 = self.case.imax if i1 is None else i1

@@ -36,7 +36,9 @@ def _dot(a, b):
 
 
 def test_transposed_filters_and_ccpml():
-    case = _euler_case()
+    # 300 x 220: several look-ahead chunks per line (the transposed sweeps run
+    # in chunks of 64)
+    case = _euler_case(300, 220)
     s = _solver(case)
     rng = np.random.default_rng(1)
     x = rng.standard_normal(s.shape)

@@ -50,3 +50,27 @@ def test_fp32_momentum_thickness():
     assert err < 1e-4
     s64.close()
     s32.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_vorticity_against_oracle(prec):
+    """omega = dv/dm - du/dn with the diffall operators (no metric): the
+    oracle's deropm/deropn with unit stretch factors are those operators."""
+    import dataclasses
+    from oracle.oracle import Oracle
+    case = config.random_case(90, 70, nk=4, D=8, seed=12, amp=1e-3, mesh_random=True)
+    s = _solver(case, prec)
+    s.run(0, 2)
+    U = s.get_state()
+    om = s.vorticity()
+    unit = dataclasses.replace(case, meshx=np.ones(case.imax), meshy=np.ones(case.jmax))
+    o = Oracle(unit)
+    ref = o.deropm(U)[..., 2] - o.deropn(U)[..., 1]
+    err = np.abs(om - ref).max() / np.abs(ref).max()
+    print(f"vorticity max rel err {err:.3e}", flush=True)
+    if prec == 0:
+        assert np.array_equal(om, ref)
+    else:
+        # the FP32 form differentiates q and the FP64 base row separately
+        assert err < 1e-12
+    s.close()
