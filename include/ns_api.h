@@ -106,6 +106,11 @@ int32_t ns_check_finite(ns_ctx *ctx);
    factors), computed on the device from the current state.  omega: host
    array of jmax x imax values, row-major (omega[j*imax + i]). */
 int32_t ns_vorticity(ns_ctx *ctx, double *omega);
+/* outt.f90:63-110 output_energy / kineticenergy_x: ec[t] = the kinetic
+   energy of column cols[t] (1-based) integrated over rows D..jmax-D,
+   (2 sum_{j=D+1}^{jmax-D-1} (u^2+v^2)/2 + the two end rows) dn/2, from the
+   current state, reduced on the device. */
+int32_t ns_station_energy(ns_ctx *ctx, int32_t nst, const int32_t *cols, double *ec);
 /* out[j] = mean over columns i0..i1 (1-based, inclusive) of component k of
    the state, j = 1..jmax (out has jmax values), reduced on the device.  For
    the statistics of the reference's mixing-layer runs (the x-averaged

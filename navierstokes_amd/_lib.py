@@ -69,6 +69,7 @@ SIGNATURES = {
     "ns_check_finite": (C.c_int32, [_V]),
     "ns_row_average": (C.c_int32, [_V, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "ns_vorticity": (C.c_int32, [_V, _DP]),
+    "ns_station_energy": (C.c_int32, [_V, C.c_int32, _IP, _DP]),
     "ns_rk_euler": (C.c_int32, [_V, C.c_int32]),
     "ns_rk_steps": (C.c_int32, [_V, C.c_int32, C.c_int32]),
     "ns_deropm": (C.c_int32, [_V, C.c_int32, _DP, _DP]),

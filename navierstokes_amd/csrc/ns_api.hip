@@ -37,6 +37,9 @@ template <typename S>
 hipError_t launch_vorticity(const S *u, const S *v, const double *ub1, int imax, int jmax, int pitch, double inv_dm,
                             double inv_dn, double *out, hipStream_t s);
 template <typename S>
+hipError_t launch_station_energy(const S *u, const S *v, const double *ub1, const double *ub2, int pitch, int jmax,
+                                 int D, double dn, int nst, const int *cols, double *out, hipStream_t s);
+template <typename S>
 hipError_t launch_row_mean(const S *U, int pitch, int jmax, int i0, int i1, const double *add, double *out,
                            hipStream_t s);
 hipError_t launch_adj_stage(const AdjPointArgs<double> &P, const AdjGatherArgs<double> &G, hipStream_t s);
@@ -943,6 +946,37 @@ int32_t ns_vorticity(ns_ctx *c, double *omega)
                                         1.0 / P.dm, 1.0 / P.dn, d, c->stream));
     }
     HIPCHK(hipMemcpyAsync(omega, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NS_OK;
+}
+
+int32_t ns_station_energy(ns_ctx *c, int32_t nst, const int32_t *cols, double *ec)
+{
+    if (!c || nst < 0 || (nst > 0 && (!cols || !ec))) return fail(NS_ERR_PARAM, "bad argument");
+    const ns_params &P = c->P;
+    if (P.D < 1 || P.jmax - 2 * P.D < 2) return fail(NS_ERR_PARAM, "the energy integral needs 1 <= D < jmax/2");
+    if (nst == 0) return NS_OK;
+    std::vector<int> c0((size_t)nst);
+    for (int t = 0; t < nst; ++t) {
+        if (cols[t] < 1 || cols[t] > P.imax) return fail(NS_ERR_PARAM, "station %d at column %d outside 1..%d", t, cols[t], P.imax);
+        c0[t] = cols[t] - 1;
+    }
+    if ((size_t)nst * (sizeof(int) + sizeof(double)) > 16 * c->plane * sizeof(double))
+        return fail(NS_ERR_PARAM, "too many stations");
+    double *dout = c->work;
+    int *dcols = reinterpret_cast<int *>(c->work + nst);
+    HIPCHK(hipMemcpyAsync(dcols, c0.data(), sizeof(int) * nst, hipMemcpyHostToDevice, c->stream));
+    if (c->pert) {
+        const float *q = static_cast<const float *>(c->S[0]);
+        const PertRows pr = pert_rows(c);
+        HIPCHK(launch_station_energy<float>(q + c->plane, q + 2 * c->plane, pr.ub[1], pr.ub[2], c->pitch, P.jmax, P.D,
+                                            P.dn, nst, dcols, dout, c->stream));
+    } else {
+        const double *U = static_cast<const double *>(c->S[0]);
+        HIPCHK(launch_station_energy<double>(U + c->plane, U + 2 * c->plane, nullptr, nullptr, c->pitch, P.jmax, P.D,
+                                             P.dn, nst, dcols, dout, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(ec, dout, sizeof(double) * nst, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return NS_OK;
 }

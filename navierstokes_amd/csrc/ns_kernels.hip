@@ -836,8 +836,10 @@ struct LineEnds {
 };
 
 // IO policy contract: load(p) for strictly increasing p (p < n),
-// emit(e, v) for strictly increasing e, finish() at the end.
-template <typename R, bool YDIR, typename IO>
+// emit(e, v) for strictly increasing e, finish() at the end.  PD: groups of
+// 13 loads issued ahead of use (2 for direct global loads, so that a load has
+// about 26 steps of work to arrive; 1 where the IO prefetches itself).
+template <typename R, bool YDIR, int PD, typename IO>
 __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b, int W, LineEnds &ends)
 {
     const bool last = (b == n);
@@ -865,18 +867,29 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
         q = q0;
     }
     const int qe = last ? n - 7 : b - 1;
-    // loads for the next 13 steps are issued one group ahead
-    R nb[13];
+    // loads for the steps of the next PD groups of 13 are in flight
+    R nb[13], nn[13];
 #pragma unroll
     for (int u = 0; u < 13; ++u) nb[u] = (q + 7 + u < n) ? io.load(q + 7 + u) : R(0);
+    if constexpr (PD == 2) {
+#pragma unroll
+        for (int u = 0; u < 13; ++u) nn[u] = (q + 20 + u < n && q + 13 + u <= qe) ? io.load(q + 20 + u) : R(0);
+    }
     int rot = 0;
     for (;;) {
         R cur[13];
 #pragma unroll
         for (int u = 0; u < 13; ++u) cur[u] = nb[u];
-        const int qn = q + 13;
+        const int qn = q + 13 * PD;
+        if constexpr (PD == 2) {
 #pragma unroll
-        for (int u = 0; u < 13; ++u) nb[u] = (qn + 7 + u < n && qn + u <= qe) ? io.load(qn + 7 + u) : R(0);
+            for (int u = 0; u < 13; ++u) nb[u] = nn[u];
+#pragma unroll
+            for (int u = 0; u < 13; ++u) nn[u] = (qn + 7 + u < n && qn + u <= qe) ? io.load(qn + 7 + u) : R(0);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 13; ++u) nb[u] = (qn + 7 + u < n && qn + u <= qe) ? io.load(qn + 7 + u) : R(0);
+        }
         bool done = false;
 #pragma unroll
         for (int u = 0; u < 13; ++u) {
@@ -936,15 +949,15 @@ struct XIO {
     R (*tout)[T + 1];
     int tc;  // start of the tile in LDS
     int oc;  // start of the output tile being filled (-1 none)
-    R pre[T];
+    R pre[T];  // the next tile, in flight (a second one costs more occupancy than it saves)
 
-    __device__ __forceinline__ void fetch(int t)  // issue loads of tile t into registers
+    __device__ __forceinline__ void fetch(R (&dst)[T], int t)  // issue loads of tile t into registers
     {
         const int c = lane % T, rs = lane / T;
 #pragma unroll
         for (int rr = 0; rr < T; ++rr) {
             const int gj = row0 + rr * RP + rs, gi = t + c;
-            pre[rr] = (gj < m && gi < n && t >= 0) ? in[(size_t)gj * pitch + gi] : R(0);
+            dst[rr] = (gj < m && gi < n && t >= 0) ? in[(size_t)gj * pitch + gi] : R(0);
         }
     }
     __device__ __forceinline__ void commit()
@@ -958,9 +971,9 @@ struct XIO {
     __device__ __forceinline__ void start(int p0)
     {
         tc = p0 & ~(T - 1);
-        fetch(tc);
+        fetch(pre, tc);
         commit();
-        fetch(tc + T);
+        fetch(pre, tc + T);
         oc = -1;
     }
     __device__ __forceinline__ R load(int p)
@@ -968,7 +981,7 @@ struct XIO {
         if (p >= tc + T) {  // uniform: all lanes stream the same positions
             commit();
             tc += T;
-            fetch(tc + T);
+            fetch(pre, tc + T);
         }
         return tin[lane][p - tc];
     }
@@ -1019,7 +1032,7 @@ __device__ __forceinline__ void filter_x_tile(const FilterArgs<R> &F, int chunk,
     const bool exact = (a <= F.W + 12);
     io.start(exact ? 0 : a - F.W - 6);
     LineEnds ends;
-    filter_line<R, false>(io, R(F.sigma), F.n, a, b, F.W, ends);
+    filter_line<R, false, 1>(io, R(F.sigma), F.n, a, b, F.W, ends);
 }
 
 template <typename R>
@@ -1070,7 +1083,7 @@ __device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblo
     io.b = b;
     io.active = col < F.m;
     LineEnds ends;
-    filter_line<R, true>(io, R(F.sigma), F.n, a, b, F.W, ends);
+    filter_line<R, true, 2>(io, R(F.sigma), F.n, a, b, F.W, ends);
     // ccpml rows j=1 and j=jmax for columns 2..imax-1 (cc.f90:46-59)
     if (F.edge_bc && io.active && col >= 1 && col <= F.m - 2) {
         R *o = F.out[k];
@@ -1420,6 +1433,53 @@ template hipError_t launch_vorticity<double>(const double *, const double *, con
 #if defined(NS_KERNELS_F32)
 template hipError_t launch_vorticity<float>(const float *, const float *, const double *, int, int, int, double,
                                             double, double *, hipStream_t);
+#endif
+
+// Kinetic energy at x-stations (outt.f90:92-110): for column i,
+//   ec = (2 sum_{j=D+1}^{jmax-D-1} k_j + k_D + k_{jmax-D}) dn / 2,
+//   k_j = (u_j^2 + v_j^2) / 2  (1-based j; the full velocity, base included)
+// One block per station, FP64 sums in a fixed order.
+template <typename S>
+__global__ __launch_bounds__(256) void station_energy_kernel(const S *u, const S *v, const double *ub1,
+                                                             const double *ub2, int pitch, int jmax, int D,
+                                                             double dn, const int *cols, double *out)
+{
+    __shared__ double part[256];
+    const int i = cols[blockIdx.x];
+    auto ke = [&](int j) {  // j: 0-based row
+        double uu = double(u[(size_t)j * pitch + i]), vv = double(v[(size_t)j * pitch + i]);
+        if (ub1) {
+            uu += ub1[j];
+            vv += ub2[j];
+        }
+        return 0.5 * (uu * uu + vv * vv);
+    };
+    double s = 0.0;
+    for (int j = D + (int)threadIdx.x; j <= jmax - D - 2; j += 256) s += ke(j);  // 1-based D+1 .. jmax-D-1
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = (2.0 * part[0] + (ke(D - 1) + ke(jmax - D - 1))) * dn / 2.0;
+}
+
+template <typename S>
+hipError_t launch_station_energy(const S *u, const S *v, const double *ub1, const double *ub2, int pitch, int jmax,
+                                 int D, double dn, int nst, const int *cols, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(station_energy_kernel<S>, dim3(nst), dim3(256), 0, s, u, v, ub1, ub2, pitch, jmax, D, dn,
+                       cols, out);
+    return hipGetLastError();
+}
+#if defined(NS_KERNELS_F64)
+template hipError_t launch_station_energy<double>(const double *, const double *, const double *, const double *, int,
+                                                  int, int, double, int, const int *, double *, hipStream_t);
+#endif
+#if defined(NS_KERNELS_F32)
+template hipError_t launch_station_energy<float>(const float *, const float *, const double *, const double *, int,
+                                                 int, int, double, int, const int *, double *, hipStream_t);
 #endif
 
 // Row averages of one plane: out[j] = mean over i in [i0, i1) of U(i, j)

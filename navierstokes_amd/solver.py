@@ -129,6 +129,14 @@ class Solver:
         _lib.check(self.lib.ns_vorticity(self.ctx, _lib.dptr(out)))
         return out
 
+    def station_energy(self, cols) -> np.ndarray:
+        """Kinetic energy of columns `cols` (1-based) integrated over y (outt.f90:92-110)."""
+        c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32).ravel())
+        out = np.empty(c.shape[0])
+        _lib.check(self.lib.ns_station_energy(self.ctx, c.shape[0], c.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              _lib.dptr(out)))
+        return out
+
     def row_average(self, k: int, i0: int = 1, i1: int | None = None) -> np.ndarray:
         """Mean of component k over columns i0..i1 (1-based, inclusive) per row."""
         i1 = self.case.imax if i1 is None else i1

@@ -74,3 +74,22 @@ def test_vorticity_against_oracle(prec):
         # the FP32 form differentiates q and the FP64 base row separately
         assert err < 1e-12
     s.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_station_energy(prec):
+    """outt.f90:92-110 kineticenergy_x at a few columns, against the same
+    sum on the state read back."""
+    case = config.mixing_layer_case(160, 96, D=10)
+    s = _solver(case, prec)
+    s.run(0, 3)
+    U = s.get_state()
+    cols = [20, 21, 80, 140]
+    got = s.station_energy(cols)
+    D, jm, dn = case.D, case.jmax, case.dn
+    for t, i in enumerate(cols):
+        u, v = U[:, i - 1, 1], U[:, i - 1, 2]
+        k = 0.5 * (u * u + v * v)
+        ec = (2.0 * k[D:jm - D - 1].sum() + (k[D - 1] + k[jm - D - 1])) * dn / 2.0
+        assert abs(got[t] - ec) <= 1e-13 * abs(ec)
+    s.close()
