@@ -867,13 +867,18 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
         q = q0;
     }
     const int qe = last ? n - 7 : b - 1;
-    // loads for the steps of the next PD groups of 13 are in flight
+    // loads for the steps of the next PD groups of 13 are in flight.  Every
+    // load is issued, at a clamped position: a value fetched past the line
+    // end or past the last step is never used, and branch-free loads let
+    // the compiler keep later loads outstanding while waiting for earlier ones.
+    const int lim = qe + 6 < n - 1 ? qe + 6 : n - 1;  // last position a refill uses
+    auto ld = [&](int p) { return io.load(p < lim ? p : lim); };
     R nb[13], nn[13];
 #pragma unroll
-    for (int u = 0; u < 13; ++u) nb[u] = (q + 7 + u < n) ? io.load(q + 7 + u) : R(0);
+    for (int u = 0; u < 13; ++u) nb[u] = ld(q + 7 + u);
     if constexpr (PD == 2) {
 #pragma unroll
-        for (int u = 0; u < 13; ++u) nn[u] = (q + 20 + u < n && q + 13 + u <= qe) ? io.load(q + 20 + u) : R(0);
+        for (int u = 0; u < 13; ++u) nn[u] = ld(q + 20 + u);
     }
     int rot = 0;
     for (;;) {
@@ -885,10 +890,10 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
 #pragma unroll
             for (int u = 0; u < 13; ++u) nb[u] = nn[u];
 #pragma unroll
-            for (int u = 0; u < 13; ++u) nn[u] = (qn + 7 + u < n && qn + u <= qe) ? io.load(qn + 7 + u) : R(0);
+            for (int u = 0; u < 13; ++u) nn[u] = ld(qn + 7 + u);
         } else {
 #pragma unroll
-            for (int u = 0; u < 13; ++u) nb[u] = (qn + 7 + u < n && qn + u <= qe) ? io.load(qn + 7 + u) : R(0);
+            for (int u = 0; u < 13; ++u) nb[u] = ld(qn + 7 + u);
         }
         bool done = false;
 #pragma unroll
@@ -953,11 +958,16 @@ struct XIO {
 
     __device__ __forceinline__ void fetch(R (&dst)[T], int t)  // issue loads of tile t into registers
     {
+        // clamped indices, every load issued: values outside the grid land in
+        // tile slots that are never read
         const int c = lane % T, rs = lane / T;
+        int gi = t + c;
+        gi = gi < 0 ? 0 : (gi < n ? gi : n - 1);
 #pragma unroll
         for (int rr = 0; rr < T; ++rr) {
-            const int gj = row0 + rr * RP + rs, gi = t + c;
-            dst[rr] = (gj < m && gi < n && t >= 0) ? in[(size_t)gj * pitch + gi] : R(0);
+            int gj = row0 + rr * RP + rs;
+            gj = gj < m ? gj : m - 1;
+            dst[rr] = in[(size_t)gj * pitch + gi];
         }
     }
     __device__ __forceinline__ void commit()
@@ -1059,7 +1069,8 @@ struct YIO {
     const double *corr;  // perturbation form: Ly(Ub) - Ub of this row (null in FP64)
     int pitch, col, a, b;
     bool active;
-    __device__ __forceinline__ R load(int p) { return active ? in[(size_t)p * pitch + col] : R(0); }
+    int colc;  // col clamped into the grid: inactive lanes load a valid value and ignore it
+    __device__ __forceinline__ R load(int p) { return in[(size_t)p * pitch + colc]; }
     __device__ __forceinline__ void emit(int e, R v)
     {
         if (active && e >= a && e < b) out[(size_t)e * pitch + col] = corr ? R(double(v) + corr[e]) : v;
@@ -1079,6 +1090,7 @@ __device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblo
     io.corr = F.corr[k];
     io.pitch = F.pitch;
     io.col = col;
+    io.colc = col < F.m ? col : F.m - 1;
     io.a = a;
     io.b = b;
     io.active = col < F.m;
