@@ -80,6 +80,25 @@ __device__ __forceinline__ R met(R d, R m)
     else return d * m;
 }
 
+// Per-row base data of the perturbation form at one row, loaded ahead by a
+// caller whose row is wave-uniform (null: read from the per-row arrays).
+template <typename R, int NK>
+struct RowCache {
+    R ub[NK], dub[NK], tb, dtb;
+};
+
+template <typename S, typename R, int NK>
+__device__ __forceinline__ void load_rows(const StageArgs<S> &A, int gy, RowCache<R, NK> &rc)
+{
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        rc.ub[k] = R(A.pr.ub[k][gy]);
+        rc.dub[k] = R(A.pr.dub[k][gy]);
+    }
+    rc.tb = A.pr.tb ? R(A.pr.tb[gy]) : R(0);
+    rc.dtb = A.pr.dtb ? R(A.pr.dtb[gy]) : R(0);
+}
+
 // level-1 slots: 0..4 = tau_xx, tau_xy, tau_yy, k dT/dx, k dT/dy (viscous),
 // then the species flux rho alpha (psi_x + psi_y) (nk = 5)
 template <int NK, bool VISC>
@@ -96,7 +115,7 @@ template <typename S, typename R, int NK, bool VISC, bool PERT, bool FAST, typen
           typename TYF>
 __device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transport<R> &tc, int gx, int gy,
                                              const UXF &ux, const UYF &uy, const TXF &tx, const TYF &ty, R tval,
-                                             R *out)
+                                             R *out, const RowCache<R, NK> *rc = nullptr)
 {
     const int imax = A.imax, jmax = A.jmax;
     const R inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn);
@@ -106,8 +125,8 @@ __device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transp
         R psy = met<R, FAST>(dv<R, FAST>(uy(4), gy, jmax, inv_dn), my);
         R rho = ux(0)(gx);
         if constexpr (PERT) {
-            psy = psy + R(A.pr.dub[4][gy]);
-            rho = rho + R(A.pr.ub[0][gy]);
+            psy = psy + (rc ? rc->dub[4] : R(A.pr.dub[4][gy]));
+            rho = rho + (rc ? rc->ub[0] : R(A.pr.ub[0][gy]));
         }
         out[L1<NK, VISC>::PSI] = rho * R(A.alpha_cond) * (psx + psy);  // derivs.f90:40
     }
@@ -119,10 +138,10 @@ __device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transp
         const R Tx = met<R, FAST>(dv<R, FAST>(tx, gx, imax, inv_dm), mx);
         R Ty = met<R, FAST>(dv<R, FAST>(ty, gy, jmax, inv_dn), my);
         if constexpr (PERT) {
-            uyv = uyv + R(A.pr.dub[1][gy]);
-            vyv = vyv + R(A.pr.dub[2][gy]);
-            Ty += R(A.pr.dtb[gy]);
-            tval += R(A.pr.tb[gy]);
+            uyv = uyv + (rc ? rc->dub[1] : R(A.pr.dub[1][gy]));
+            vyv = vyv + (rc ? rc->dub[2] : R(A.pr.dub[2][gy]));
+            Ty += rc ? rc->dtb : R(A.pr.dtb[gy]);
+            tval += rc ? rc->tb : R(A.pr.tb[gy]);
         }
         R mu, kk;
         transport_eval<R>(tc, tval, mu, kk);
@@ -142,7 +161,8 @@ __device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transp
 template <typename S, typename R, int NK, bool VISC, bool PERT, bool FAST, typename UXF, typename UYF, typename LXF,
           typename LYF, typename LVF>
 __device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy, const UXF &ux, const UYF &uy,
-                                          const LXF &lx, const LYF &ly, const LVF &lv, R *qc, R *uc, R *f)
+                                          const LXF &lx, const LYF &ly, const LVF &lv, R *qc, R *uc, R *f,
+                                          const RowCache<R, NK> *rc = nullptr)
 {
     const int imax = A.imax, jmax = A.jmax;
     const R inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn), gam = R(A.gamma);
@@ -155,8 +175,8 @@ __device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy,
         fx[k] = met<R, FAST>(dv<R, FAST>(ux(k), gx, imax, inv_dm), mx);
         fy[k] = met<R, FAST>(dv<R, FAST>(uy(k), gy, jmax, inv_dn), my);
         if constexpr (PERT) {
-            uc[k] = qc[k] + R(A.pr.ub[k][gy]);
-            fy[k] = fy[k] + R(A.pr.dub[k][gy]);
+            uc[k] = qc[k] + (rc ? rc->ub[k] : R(A.pr.ub[k][gy]));
+            fy[k] = fy[k] + (rc ? rc->dub[k] : R(A.pr.dub[k][gy]));
         }
     }
     const R rho = uc[0], u = uc[1], v = uc[2], p = uc[3];
@@ -450,7 +470,10 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
             }
         }
     };
-    // accessors (r, gy: wave-uniform rows)
+    // accessors.  r, r0: wave-uniform rows; a y-accessor centred on row r0
+    // takes the slot of r0 once and reaches rows r0 +- 6 by a wrapped offset
+    // (the offsets are compile-time constants after inlining)
+    auto wrap = [](int s, int n) { return s < 0 ? s + n : (s >= n ? s - n : s); };
     auto ux_at = [=](int r) {
         const int s = (r - wy0) % WR;
         return [=](int k) {
@@ -458,28 +481,30 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
             return [row, ox](int q) { return row[q - ox]; };
         };
     };
-    auto uy_at = [=](int gx) {
+    auto uy_at = [=](int gx, int r0) {
+        const int s0 = (r0 - wy0) % WR;
         return [=](int k) {
             const R *col = sU + k * WR * MWX + (gx - ox);
-            return [col, wy0](int q) { return col[((q - wy0) % WR) * MWX]; };
+            return [=](int q) { return col[wrap(s0 + (q - r0), WR) * MWX]; };
         };
     };
     auto tx_at = [=](int r) {
         const R *row = sT + ((r - wy0) % WR) * MWX;
         return [row, ox](int q) { return row[q - ox]; };
     };
-    auto ty_at = [=](int gx) {
+    auto ty_at = [=](int gx, int r0) {
         const R *col = sT + (gx - ox);
-        return [col, wy0](int q) { return col[((q - wy0) % WR) * MWX]; };
+        const int s0 = (r0 - wy0) % WR;
+        return [=](int q) { return col[wrap(s0 + (q - r0), WR) * MWX]; };
     };
     const Transport<R> tc{R(A.tc.A_mi), R(A.tc.Bt_mi), R(A.tc.Ct_mi), R(A.tc.D_mi), R(A.tc.inv_mi0),
                           R(A.tc.A_k),  R(A.tc.Bt_k),  R(A.tc.Ct_k),  R(A.tc.D_k),  R(A.tc.inv_k0),
                           R(A.tc.Tref)};
-    auto level1_row = [&](int r) {  // wave-uniform row r, column lx0 + lane
+    auto level1_row = [&](int r, const RowCache<R, NK> *rc) {  // wave-uniform row r, column lx0 + lane
         const int gx = lx0 + lane;
         R out[NL > 0 ? NL : 1];
-        level1_point<S, R, NK, VISC, PERT, FAST>(A, tc, gx, r, ux_at(r), uy_at(gx), tx_at(r), ty_at(gx),
-                                                  VISC ? tx_at(r)(gx) : R(0), out);
+        level1_point<S, R, NK, VISC, PERT, FAST>(A, tc, gx, r, ux_at(r), uy_at(gx, r), tx_at(r), ty_at(gx, r),
+                                                  VISC ? tx_at(r)(gx) : R(0), out, rc);
         const int s = (r - ly0) % LR;
 #pragma unroll
         for (int l = 0; l < NL; ++l) sL[(l * LR + s) * MLX + lane] = out[l];
@@ -494,7 +519,7 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
     }
     __syncthreads();
     if constexpr (NL > 0) {
-        for (int rr = wave; rr < 6; rr += RB) level1_row(ly0 + rr);
+        for (int rr = wave; rr < 6; rr += RB) level1_row(ly0 + rr, nullptr);
     }
     R pre[PF][NK];
 #pragma unroll
@@ -504,7 +529,13 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
     }
 
     for (int base = y0; base < y1; base += RB) {
-        // W of this step's outputs, loaded now so that the loads overlap (a) and (b)
+        // per-row base data of this step's level-1 and output rows, and W of
+        // this step's outputs, loaded now so that the loads overlap (a) and (b)
+        RowCache<R, NK> rcl, rco;
+        if constexpr (PERT) {
+            load_rows<S, R, NK>(A, base + 3 + wave, rcl);
+            load_rows<S, R, NK>(A, base + wave, rco);
+        }
         S wpre[NK];
         {
             const size_t g = (size_t)(base + wave) * pitch + (i0 + lane);
@@ -527,7 +558,7 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
         }
         // (b) level-1 row base+3+wave
         if constexpr (NL > 0) {
-            level1_row(base + 3 + wave);
+            level1_row(base + 3 + wave, PERT ? &rcl : nullptr);
             __syncthreads();
         }
         // (c) outputs of row base+wave
@@ -541,11 +572,12 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
             };
             auto lya = [=](int l) {
                 const R *col = sL + l * LR * MLX + (gx - lx0);
-                return [col, ly0](int q) { return col[((q - ly0) % LR) * MLX]; };
+                return [=](int q) { return col[wrap(ls + (q - gy), LR) * MLX]; };
             };
             auto lv = [=](int l) { return sL[(l * LR + ls) * MLX + (gx - lx0)]; };
             R qc[NK], uc[NK], f[NK];
-            rhs_point<S, R, NK, VISC, PERT, FAST>(A, gx, gy, ux_at(gy), uy_at(gx), lxa, lya, lv, qc, uc, f);
+            rhs_point<S, R, NK, VISC, PERT, FAST>(A, gx, gy, ux_at(gy), uy_at(gx, gy), lxa, lya, lv, qc, uc, f,
+                                                  PERT ? &rco : nullptr);
             stage_store<S, R, NK, false, PERT, FAST>(A, gx, gy, qc, uc, f, wpre);
         }
     }
