@@ -346,6 +346,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=3)
+    ap.add_argument("--variant", type=int, default=-1, help="stage variant 0..2 (default: the autotuned one)")
     ap.add_argument("--precision", choices=["fp64", "fp32pert"], default=None,
                     help="default fp64 (ns, pulse) / fp32pert (batch)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -379,6 +380,9 @@ def main():
     print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
     s = Solver(case, precision=prec)
     variant, tune_ms = s.autotune()
+    if args.variant >= 0:
+        s.set_variant(args.variant)
+        variant = args.variant
     s.run(0, args.warmup)
     s.synchronize()
     print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)

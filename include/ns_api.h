@@ -203,14 +203,22 @@ int32_t ns_vjp_step(ns_ctx *ctx, int32_t iter, const double *U, double *lam);
 
 /* tuning ---------------------------------------------------------------------- */
 /* Time the stage launch's variants on this context's workload and keep the
-   fastest; results are bitwise identical across variants.  FP64: the
-   interior marching kernel at 4 or 8 rows per step, the edge tiles beside it
-   on a second stream or after it; FP32: interior tiles of 256, 512 or 1024
-   threads.  Runs by itself before the first step; call it between steps
-   only.  variant (may be NULL) gets the choice 0..2, ms[3] (may be NULL) the
+   fastest.  FP64 (results bitwise identical across variants): 0 = the
+   interior marching kernel, 4 rows per step, over the whole region 9 points
+   in from the edges (buffers and metrics point by point; blocks inside the
+   fast zone, a unit grid outside the buffer/metric layers, take the fast
+   code) with a thin edge band beside it on a second stream; 1 = the marching
+   kernel over the fast zone only, the edge ring beside it; 2 = as 0 at 8
+   rows per step.  FP32: 0 = interior tiles, 1 = as FP64's 0, 2 = as FP64's
+   1 (the tile and marching kernels may round differently).
+   Runs by itself before the first step; call it between steps only.
+   variant (may be NULL) gets the choice 0..2, ms[3] (may be NULL) the
    average time of each variant (-1 if it could not launch).
    No reference counterpart: the reference has no tunable kernels. */
 int32_t ns_autotune(ns_ctx *ctx, int32_t *variant, double *ms);
+/* Use stage variant 0..2 (as ns_autotune) without timing.
+   No reference counterpart. */
+int32_t ns_set_variant(ns_ctx *ctx, int32_t variant);
 
 /* measurement ----------------------------------------------------------------- */
 /* average device time (ms) of each kernel class over the last ns_rk_steps

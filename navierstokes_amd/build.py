@@ -69,10 +69,16 @@ def build(verbose: bool = False) -> Path:
         ("ns_adjoint.hip", "ns_adjoint_f32.o", ["-DNS_KERNELS_F32"]),
         ("ns_api.hip", "ns_api.o", []),
     ]
+    # the units are independent: compile them concurrently
+    from concurrent.futures import ThreadPoolExecutor
+    cmds = []
     for src, obj, extra in units:
         out = bdir / obj
-        _run([cc, *base, *extra, *inc, "-c", str(CSRC / src), "-o", str(out)], verbose)
+        cmds.append([cc, *base, *extra, *inc, "-c", str(CSRC / src), "-o", str(out)])
         objs.append(str(out))
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)], verbose)
     os.replace(tmp, LIB)

@@ -639,6 +639,8 @@ int set_ptr_table(ns_ctx *c, int slot, void *buf, int nk, size_t esz)
     void *h[8];
     for (int k = 0; k < 8; ++k) h[k] = k < nk ? static_cast<char *>(buf) + (size_t)k * c->plane * esz : nullptr;
     HIPCHK(hipMemcpyAsync(c->dptrs + 8 * slot, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    // h lives on this stack frame: the copy must have read it before return
+    HIPCHK(hipStreamSynchronize(c->stream));
     return NS_OK;
 }
 
@@ -1202,6 +1204,19 @@ int32_t ns_autotune(ns_ctx *c, int32_t *variant, double *ms)
     if (variant) *variant = c->stage_variant;
     if (ms)
         for (int v = 0; v < 3; ++v) ms[v] = c->tune_ms[v];
+    return NS_OK;
+}
+
+int32_t ns_set_variant(ns_ctx *c, int32_t variant)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (variant < 0 || variant > 2) return fail(NS_ERR_PARAM, "stage variant must be 0, 1 or 2");
+    if (c->graph) {
+        HIPCHK(hipGraphExecDestroy(c->graph));
+        c->graph = nullptr;
+    }
+    c->stage_variant = variant;
     return NS_OK;
 }
 

@@ -96,6 +96,16 @@ __device__ __forceinline__ R drp_right_row(const F &f, int n, R inv_h)
     return s;
 }
 
+// Interior DRP first derivative at q (3 <= q <= n-4), diff.f90:40-47.
+template <typename R, typename F>
+__device__ __forceinline__ R drp_interior(const F &f, int q, R inv_h)
+{
+    const R t1 = R(DRP_A1) * (f(q + 1) - f(q - 1));
+    const R t2 = R(DRP_A2) * (f(q + 2) - f(q - 2));
+    const R t3 = R(DRP_A3) * (f(q + 3) - f(q - 3));
+    return (t1 * inv_h + t2 * inv_h) + t3 * inv_h;
+}
+
 // First derivative at position q of a line of n points (0-based), the line
 // read through accessor f(pos).  Interior: DRP (diff.f90:40-47); positions
 // 0..2 and n-3..n-1: one-sided 7-point stencils (diff.f90:49-125).  The
@@ -103,12 +113,7 @@ __device__ __forceinline__ R drp_right_row(const F &f, int n, R inv_h)
 template <typename R, typename F>
 __device__ __forceinline__ R drp_deriv(const F &f, int q, int n, R inv_h)
 {
-    if (q >= 3 && q <= n - 4) {
-        const R t1 = R(DRP_A1) * (f(q + 1) - f(q - 1));
-        const R t2 = R(DRP_A2) * (f(q + 2) - f(q - 2));
-        const R t3 = R(DRP_A3) * (f(q + 3) - f(q - 3));
-        return (t1 * inv_h + t2 * inv_h) + t3 * inv_h;
-    }
+    if (q >= 3 && q <= n - 4) return drp_interior<R>(f, q, inv_h);
     if (q == 0) return drp_left_row<0, R>(f, inv_h);
     if (q == 1) return drp_left_row<1, R>(f, inv_h);
     if (q == 2) return drp_left_row<2, R>(f, inv_h);

@@ -149,9 +149,20 @@ struct RectSet {
 
 // marching region: output columns [xa, xa + 58 nstrips), rows [ya, yb) in
 // chunks of H rows (blockIdx.x = strip, blockIdx.y = chunk)
+// Rows [ya, yb) are marched as: one chunk [ya, y_a) (if not empty), chunks of
+// H rows over [y_a, y_b), one chunk [y_b, yb) (if not empty).  Hybrid layout
+// (fx1 > fx0): blocks whose outputs lie in [fx0, fx1) x [y_a, y_b) take the
+// fast per-point code.
 struct MarchMap {
     int xa, nstrips, ya, yb, H;
+    int y_a, y_b;
+    int fx0, fx1;
 };
+
+__host__ __device__ inline int march_chunks(const MarchMap &m)
+{
+    return (m.y_a > m.ya ? 1 : 0) + (m.y_b - m.y_a + m.H - 1) / m.H + (m.yb > m.y_b ? 1 : 0);
+}
 
 // Output columns / rows of the marching region for strips of 58 and steps of
 // rb rows, rows a multiple of rb.  fast: every output's 3-point level-1 halo
@@ -169,7 +180,29 @@ __host__ __device__ inline bool march_region(bool fast, int D, int imax, int jma
     const int rows = jmax - lo - m.ya;
     m.yb = m.ya + (rows > 0 ? rows / rb * rb : 0);
     m.H = 0;
+    m.y_a = m.ya;
+    m.y_b = m.yb;
+    m.fx0 = m.fx1 = 0;
     return m.nstrips >= 1 && m.yb - m.ya >= 4 * rb;
+}
+
+// General region with the fast zone marked (unit grids outside the layers):
+// columns as the general region; rows split at the fast zone's bounds, each
+// part a multiple of rb.  False when there is no fast zone of at least 4 rb
+// rows; then the plain general region applies.
+__host__ __device__ inline bool march_hybrid(int D, int imax, int jmax, int rb, MarchMap &m)
+{
+    MarchMap f;
+    if (!march_region(true, D, imax, jmax, rb, f) || !march_region(false, D, imax, jmax, rb, m)) return false;
+    const int lo = f.ya;            // fast zone rows/columns [lo, n - lo)
+    m.ya = lo >= 9 ? 9 + (lo - 9) % rb : lo;  // lo - ya: a multiple of rb
+    m.y_a = lo;
+    m.y_b = lo + (jmax - 2 * lo) / rb * rb;
+    const int top = jmax - (lo >= 9 ? 9 : lo) - m.y_b;
+    m.yb = m.y_b + (top > 0 ? top / rb * rb : 0);
+    m.fx0 = lo;
+    m.fx1 = imax - lo;
+    return m.y_b - m.y_a >= 4 * rb;
 }
 
 }  // namespace nsamd

@@ -60,23 +60,31 @@ __device__ __forceinline__ R drp_interior_unit(const F &f, int q)
 
 // ---------------------------------------------------------------------------
 // Per-point pieces of the stage, shared by the tile and the marching kernels,
-// which therefore compute every value with the same operations.
-// FAST: the point and its 3-point level-1 halo lie in the interior, outside
-// the metric/buffer layers, on a unit grid.  Then no one-sided stencil, no
-// metric factor (it is exactly 1), no buffer and no edge test is needed, and
-// dropping the multiplications by 1.0 leaves every result bit unchanged.
+// which therefore compute every value with the same operations.  A point
+// mode PM says what the caller guarantees about the points it computes:
+//   PM_GEN : nothing (one-sided stencils near the edges, metric factors,
+//            buffer layers and the edge test point by point);
+//   PM_MID : the point and its 3-point level-1 halo lie at least 6 points
+//            from every edge, so every stencil is the interior one;
+//   PM_MIDU: PM_MID on a unit grid (dm = dn = 1);
+//   PM_FAST: PM_MIDU outside the metric/buffer layers.
+// The stronger modes drop branches and multiplications by exactly 1.0, so
+// every result bit is the one PM_GEN computes.
 // ---------------------------------------------------------------------------
-template <typename R, bool FAST, typename F>
+constexpr int PM_GEN = 0, PM_MID = 1, PM_MIDU = 2, PM_FAST = 3;
+
+template <typename R, int PM, typename F>
 __device__ __forceinline__ R dv(const F &f, int q, int n, R inv_h)
 {
-    if constexpr (FAST) return drp_interior_unit<R>(f, q);
+    if constexpr (PM >= PM_MIDU) return drp_interior_unit<R>(f, q);
+    else if constexpr (PM == PM_MID) return drp_interior<R>(f, q, inv_h);
     else return drp_deriv<R>(f, q, n, inv_h);
 }
 
-template <typename R, bool FAST>
+template <typename R, int PM>
 __device__ __forceinline__ R met(R d, R m)
 {
-    if constexpr (FAST) return d;
+    if constexpr (PM == PM_FAST) return d;
     else return d * m;
 }
 
@@ -107,22 +115,22 @@ struct L1 {
     static constexpr int N = (VISC ? 5 : 0) + (NK >= 5 ? 1 : 0);
 };
 
-// Level-1 fields at (gx, gy) (eqeuler.f90:34-41, derivs.f90:25-56, :40).
+// Level-1 fields at (gx, gy) (eqeuler.f90:34-41, derivs.f90:25-56, :40);
+// mx, my: the metric factors there (metx[gx], mety[gy]; unused in PM_FAST).
 // ux(k) / uy(k): accessor of stored component k along x at row gy / along y
 // at column gx; tx / ty: the same for T (T - Tb in the perturbation form);
 // tval: that value at the point.
-template <typename S, typename R, int NK, bool VISC, bool PERT, bool FAST, typename UXF, typename UYF, typename TXF,
+template <typename S, typename R, int NK, bool VISC, bool PERT, int PM, typename UXF, typename UYF, typename TXF,
           typename TYF>
-__device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transport<R> &tc, int gx, int gy,
-                                             const UXF &ux, const UYF &uy, const TXF &tx, const TYF &ty, R tval,
-                                             R *out, const RowCache<R, NK> *rc = nullptr)
+__device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transport<R> &tc, int gx, int gy, R mx,
+                                             R my, const UXF &ux, const UYF &uy, const TXF &tx, const TYF &ty,
+                                             R tval, R *out, const RowCache<R, NK> *rc = nullptr)
 {
     const int imax = A.imax, jmax = A.jmax;
     const R inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn);
-    const R mx = FAST ? R(1) : R(A.metx[gx]), my = FAST ? R(1) : R(A.mety[gy]);
     if constexpr (NK >= 5) {
-        const R psx = met<R, FAST>(dv<R, FAST>(ux(4), gx, imax, inv_dm), mx);
-        R psy = met<R, FAST>(dv<R, FAST>(uy(4), gy, jmax, inv_dn), my);
+        const R psx = met<R, PM>(dv<R, PM>(ux(4), gx, imax, inv_dm), mx);
+        R psy = met<R, PM>(dv<R, PM>(uy(4), gy, jmax, inv_dn), my);
         R rho = ux(0)(gx);
         if constexpr (PERT) {
             psy = psy + (rc ? rc->dub[4] : R(A.pr.dub[4][gy]));
@@ -131,12 +139,12 @@ __device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transp
         out[L1<NK, VISC>::PSI] = rho * R(A.alpha_cond) * (psx + psy);  // derivs.f90:40
     }
     if constexpr (VISC) {
-        const R uxv = met<R, FAST>(dv<R, FAST>(ux(1), gx, imax, inv_dm), mx);
-        const R vxv = met<R, FAST>(dv<R, FAST>(ux(2), gx, imax, inv_dm), mx);
-        R uyv = met<R, FAST>(dv<R, FAST>(uy(1), gy, jmax, inv_dn), my);
-        R vyv = met<R, FAST>(dv<R, FAST>(uy(2), gy, jmax, inv_dn), my);
-        const R Tx = met<R, FAST>(dv<R, FAST>(tx, gx, imax, inv_dm), mx);
-        R Ty = met<R, FAST>(dv<R, FAST>(ty, gy, jmax, inv_dn), my);
+        const R uxv = met<R, PM>(dv<R, PM>(ux(1), gx, imax, inv_dm), mx);
+        const R vxv = met<R, PM>(dv<R, PM>(ux(2), gx, imax, inv_dm), mx);
+        R uyv = met<R, PM>(dv<R, PM>(uy(1), gy, jmax, inv_dn), my);
+        R vyv = met<R, PM>(dv<R, PM>(uy(2), gy, jmax, inv_dn), my);
+        const R Tx = met<R, PM>(dv<R, PM>(tx, gx, imax, inv_dm), mx);
+        R Ty = met<R, PM>(dv<R, PM>(ty, gy, jmax, inv_dn), my);
         if constexpr (PERT) {
             uyv = uyv + (rc ? rc->dub[1] : R(A.pr.dub[1][gy]));
             vyv = vyv + (rc ? rc->dub[2] : R(A.pr.dub[2][gy]));
@@ -156,24 +164,23 @@ __device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transp
 }
 
 // RHS at (gx, gy) (eqeuler.f90:61-97): f[], the stored state qc[] and the
-// full state uc[] there.  lx(l) / ly(l): accessor of level-1 field l along x
+// full state uc[] there; mx, my as for level1_point.  lx(l) / ly(l): accessor of level-1 field l along x
 // at row gy / along y at column gx; lv(l): its value at the point.
-template <typename S, typename R, int NK, bool VISC, bool PERT, bool FAST, typename UXF, typename UYF, typename LXF,
+template <typename S, typename R, int NK, bool VISC, bool PERT, int PM, typename UXF, typename UYF, typename LXF,
           typename LYF, typename LVF>
-__device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy, const UXF &ux, const UYF &uy,
-                                          const LXF &lx, const LYF &ly, const LVF &lv, R *qc, R *uc, R *f,
-                                          const RowCache<R, NK> *rc = nullptr)
+__device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy, R mx, R my, const UXF &ux,
+                                          const UYF &uy, const LXF &lx, const LYF &ly, const LVF &lv, R *qc, R *uc,
+                                          R *f, const RowCache<R, NK> *rc = nullptr)
 {
     const int imax = A.imax, jmax = A.jmax;
     const R inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn), gam = R(A.gamma);
-    const R mx = FAST ? R(1) : R(A.metx[gx]), my = FAST ? R(1) : R(A.mety[gy]);
     R fx[NK], fy[NK];
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
         qc[k] = ux(k)(gx);
         uc[k] = qc[k];
-        fx[k] = met<R, FAST>(dv<R, FAST>(ux(k), gx, imax, inv_dm), mx);
-        fy[k] = met<R, FAST>(dv<R, FAST>(uy(k), gy, jmax, inv_dn), my);
+        fx[k] = met<R, PM>(dv<R, PM>(ux(k), gx, imax, inv_dm), mx);
+        fy[k] = met<R, PM>(dv<R, PM>(uy(k), gy, jmax, inv_dn), my);
         if constexpr (PERT) {
             uc[k] = qc[k] + (rc ? rc->ub[k] : R(A.pr.ub[k][gy]));
             fy[k] = fy[k] + (rc ? rc->dub[k] : R(A.pr.dub[k][gy]));
@@ -185,12 +192,12 @@ __device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy,
     // eqeuler.f90:63-66
     f[0] = -(u * fx[0] + rho * fx[1] + v * fy[0] + rho * fy[2]);
     if constexpr (VISC) {
-        const R txx_x = met<R, FAST>(dv<R, FAST>(lx(0), gx, imax, inv_dm), mx);
-        const R txy_x = met<R, FAST>(dv<R, FAST>(lx(1), gx, imax, inv_dm), mx);
-        const R ktx_x = met<R, FAST>(dv<R, FAST>(lx(3), gx, imax, inv_dm), mx);
-        const R txy_y = met<R, FAST>(dv<R, FAST>(ly(1), gy, jmax, inv_dn), my);
-        const R tyy_y = met<R, FAST>(dv<R, FAST>(ly(2), gy, jmax, inv_dn), my);
-        const R kty_y = met<R, FAST>(dv<R, FAST>(ly(4), gy, jmax, inv_dn), my);
+        const R txx_x = met<R, PM>(dv<R, PM>(lx(0), gx, imax, inv_dm), mx);
+        const R txy_x = met<R, PM>(dv<R, PM>(lx(1), gx, imax, inv_dm), mx);
+        const R ktx_x = met<R, PM>(dv<R, PM>(lx(3), gx, imax, inv_dm), mx);
+        const R txy_y = met<R, PM>(dv<R, PM>(ly(1), gy, jmax, inv_dn), my);
+        const R tyy_y = met<R, PM>(dv<R, PM>(ly(2), gy, jmax, inv_dn), my);
+        const R kty_y = met<R, PM>(dv<R, PM>(ly(4), gy, jmax, inv_dn), my);
         const R t0 = lv(0), t1 = lv(1), t2 = lv(2);
         const R vre = R(A.inv_Re) * irho;
         // eqeuler.f90:68-92
@@ -206,18 +213,48 @@ __device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy,
     }
     if constexpr (NK >= 5) {
         constexpr int P = L1<NK, VISC>::PSI;
-        const R psd_x = met<R, FAST>(dv<R, FAST>(lx(P), gx, imax, inv_dm), mx);
-        const R psd_y = met<R, FAST>(dv<R, FAST>(ly(P), gy, jmax, inv_dn), my);
+        const R psd_x = met<R, PM>(dv<R, PM>(lx(P), gx, imax, inv_dm), mx);
+        const R psd_y = met<R, PM>(dv<R, PM>(ly(P), gy, jmax, inv_dn), my);
         f[4] = -(u * fx[4] + v * fy[4] - R(A.inv_Pe) * irho * (psd_x + psd_y));  // eqeuler.f90:94-95
     }
 }
 
+// Data of one output point loaded ahead by a caller: W, and at buffer
+// points the buffer sigmas and (FP64 form) the base state.
+template <typename S, int NK>
+struct StorePre {
+    S w[NK];
+    S b[NK];
+    double sx, sy;
+};
+
+// W is loaded unconditionally (gx, gy must lie in the grid) so that the
+// number of loads in flight is known to the compiler.
+template <typename S, int NK, bool PERT, int PM>
+__device__ __forceinline__ void store_prefetch(const StageArgs<S> &A, int gx, int gy, StorePre<S, NK> &p)
+{
+    const size_t g = (size_t)gy * A.pitch + gx;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const S w = A.W[k][g];
+        p.w[k] = A.first ? S(0) : w;
+    }
+    if constexpr (PM != PM_FAST) {
+        const bool ybuf = gy < A.D || gy >= A.jmax - A.D;
+        const bool xbuf = gx < A.D || gx >= A.imax - A.D;
+        p.sy = ybuf ? A.sigy[gy] : 1.0;
+        p.sx = xbuf ? A.sigx[gx] : 1.0;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) p.b[k] = (!PERT && (ybuf || xbuf)) ? A.base[k][g] : S(0);
+    }
+}
+
 // LDDRK update at interior points (rk.f90:160-167) and the buffer layers
-// (rk.f90:81-108), or the RHS alone (RHS_ONLY).  wpre: W at the point, loaded
-// ahead by the caller (null: load it here).
-template <typename S, typename R, int NK, bool RHS_ONLY, bool PERT, bool FAST>
+// (rk.f90:81-108), or the RHS alone (RHS_ONLY).  pre: the point's data
+// loaded ahead by store_prefetch (null: load it here).
+template <typename S, typename R, int NK, bool RHS_ONLY, bool PERT, int PM>
 __device__ __forceinline__ void stage_store(const StageArgs<S> &A, int gx, int gy, const R *qc, const R *uc,
-                                            const R *f, const S *wpre = nullptr)
+                                            const R *f, const StorePre<S, NK> *pre = nullptr)
 {
     const int imax = A.imax, jmax = A.jmax;
     const size_t g = (size_t)gy * A.pitch + gx;
@@ -226,36 +263,36 @@ __device__ __forceinline__ void stage_store(const StageArgs<S> &A, int gx, int g
         for (int k = 0; k < NK; ++k) A.dudt[k][g] = S(f[k]);
         return;
     }
-    const bool interior = FAST || (gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2);
-    const bool ybuf = !FAST && (gy < A.D || gy >= jmax - A.D);
-    const bool xbuf = !FAST && (gx < A.D || gx >= imax - A.D);
+    const bool interior = PM != PM_GEN || (gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2);
+    const bool ybuf = PM != PM_FAST && (gy < A.D || gy >= jmax - A.D);
+    const bool xbuf = PM != PM_FAST && (gx < A.D || gx >= imax - A.D);
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
         if constexpr (PERT) {
             // stage accumulation of the state in FP64 registers
             double qn = double(qc[k]);
             if (interior) {
-                const double w0 = double(wpre ? wpre[k] : A.W[k][g]);
+                const double w0 = double(pre ? pre->w[k] : A.W[k][g]);
                 const double wn = A.first ? A.dt * double(f[k]) : A.alpha * w0 + A.dt * double(f[k]);
                 if (!A.last) A.W[k][g] = S(wn);
                 qn = qn + A.beta * wn;
             }
             // Ub + s (U - Ub) - Ub = s q
-            if (ybuf) qn = A.sigy[gy] * qn;
-            if (xbuf) qn = A.sigx[gx] * qn;
+            if (ybuf) qn = (pre ? pre->sy : A.sigy[gy]) * qn;
+            if (xbuf) qn = (pre ? pre->sx : A.sigx[gx]) * qn;
             A.Uout[k][g] = S(qn);
         } else {
             R un = uc[k];
             if (interior) {
-                const R w0 = R(wpre ? wpre[k] : A.W[k][g]);
+                const R w0 = R(pre ? pre->w[k] : A.W[k][g]);
                 const R wn = A.first ? R(A.dt) * f[k] : R(A.alpha) * w0 + R(A.dt) * f[k];
                 if (!A.last) A.W[k][g] = S(wn);
                 un = uc[k] + R(A.beta) * wn;
             }
             if (ybuf || xbuf) {  // y layer first, then x
-                const R ub = R(A.base[k][g]);
-                if (ybuf) un = ub + R(A.sigy[gy]) * (un - ub);
-                if (xbuf) un = ub + R(A.sigx[gx]) * (un - ub);
+                const R ub = R(pre ? pre->b[k] : A.base[k][g]);
+                if (ybuf) un = ub + R(pre ? pre->sy : A.sigy[gy]) * (un - ub);
+                if (xbuf) un = ub + R(pre ? pre->sx : A.sigx[gx]) * (un - ub);
             }
             A.Uout[k][g] = S(un);
         }
@@ -267,7 +304,7 @@ __device__ __forceinline__ void stage_store(const StageArgs<S> &A, int gx, int g
 // STX x TY) from an LDS box holding U (and T) on the tile plus a 6-point halo
 // and the level-1 fields on the tile plus a 3-point halo.
 // ---------------------------------------------------------------------------
-template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT, bool FAST>
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT, int PM>
 __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1, int j0, int j1)
 {
     constexpr int TX = STX;
@@ -335,8 +372,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1
             const R *trow = sT + (gy - oy) * WX;
             const R *tcol = sT + (gx - ox);
             R out[NL];
-            level1_point<S, R, NK, VISC, PERT, FAST>(
-                A, tc, gx, gy, ux_at(gy), uy_at(gx), [trow, ox](int q) { return trow[q - ox]; },
+            level1_point<S, R, NK, VISC, PERT, PM>(
+                A, tc, gx, gy, PM == PM_FAST ? R(1) : R(A.metx[gx]), PM == PM_FAST ? R(1) : R(A.mety[gy]),
+                ux_at(gy), uy_at(gx), [trow, ox](int q) { return trow[q - ox]; },
                 [tcol, oy](int q) { return tcol[(q - oy) * WX]; }, VISC ? trow[gx - ox] : R(0), out);
             const int o = ly * LX + lx;
 #pragma unroll
@@ -360,8 +398,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1
         };
         auto lv = [=](int l) { return sL[l * LY * LX + (gy - ly0) * LX + (gx - lx0)]; };
         R qc[NK], uc[NK], f[NK];
-        rhs_point<S, R, NK, VISC, PERT, FAST>(A, gx, gy, ux_at(gy), uy_at(gx), lxa, lya, lv, qc, uc, f);
-        stage_store<S, R, NK, RHS_ONLY, PERT, FAST>(A, gx, gy, qc, uc, f);
+        rhs_point<S, R, NK, VISC, PERT, PM>(A, gx, gy, PM == PM_FAST ? R(1) : R(A.metx[gx]),
+                                            PM == PM_FAST ? R(1) : R(A.mety[gy]), ux_at(gy), uy_at(gx), lxa, lya,
+                                            lv, qc, uc, f);
+        stage_store<S, R, NK, RHS_ONLY, PERT, PM>(A, gx, gy, qc, uc, f);
     }
 }
 
@@ -370,7 +410,7 @@ template <typename S, typename R, int NK, bool VISC, int TY, bool PERT, int NT>
 __global__ __launch_bounds__(NT) void stage_kernel_fast(StageArgs<S> A, int bx0, int by0)
 {
     const int i0 = (bx0 + (int)blockIdx.x) * STX, j0 = (by0 + (int)blockIdx.y) * TY;
-    stage_tile<S, R, NK, VISC, TY, false, PERT, NT, true>(A, i0, i0 + STX, j0, j0 + TY);
+    stage_tile<S, R, NK, VISC, TY, false, PERT, NT, PM_FAST>(A, i0, i0 + STX, j0, j0 + TY);
 }
 
 // general tiles over up to four rectangles (the grid around the interior region)
@@ -379,7 +419,7 @@ __global__ __launch_bounds__(NT) void stage_kernel_rects(StageArgs<S> A, RectSet
 {
     int i0, i1, j0, j1;
     rs.tile(blockIdx.x, i0, i1, j0, j1);
-    stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, false>(A, i0, i1, j0, j1);
+    stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, PM_GEN>(A, i0, i1, j0, j1);
 }
 
 // Batched launch: one block per entry of a tile table spanning every case;
@@ -393,12 +433,14 @@ __global__ __launch_bounds__(NT) void stage_kernel_batch(const StageArgs<S> *arg
     const int4 t = tiles[blockIdx.x];
     const StageArgs<S> &A = args[(size_t)stage * ncases + t.x];
     const int i0 = t.y * STX, j0 = t.z * TY;
-    stage_tile<S, R, NK, VISC, TY, false, PERT, NT, FAST>(A, i0, min(i0 + STX, A.imax), j0, min(j0 + TY, A.jmax));
+    stage_tile<S, R, NK, VISC, TY, false, PERT, NT, FAST ? PM_FAST : PM_GEN>(A, i0, min(i0 + STX, A.imax), j0,
+                                                                        min(j0 + TY, A.jmax));
 }
 
 // ---------------------------------------------------------------------------
-// Marching version (interior region; FAST: the fast zone, else the general
-// region of march_region with buffers and metrics point by point).  A block
+// Marching version (interior region; PM_FAST: the fast zone, else the general
+// region of march_region, PM_MID(U), with buffers and metrics point by
+// point).  A block
 // owns a strip of MTX = 58
 // output columns, so its level-1 rows are exactly 64 wide (one wave per row)
 // and its U window rows 70 wide, and marches up a chunk of rows RB at a time
@@ -431,8 +473,9 @@ static size_t march_smem(int NK, bool visc, int RB)
     return sizeof(R) * ((size_t)(NK + (visc ? 1 : 0)) * WR * MWX + (size_t)NL * LR * MLX);
 }
 
-template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, bool FAST>
-__global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, MarchMap m)
+// one block's march: output columns [i0, i0 + 58), rows [y0, y1)
+template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
+__device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, const int y0, const int y1)
 {
     constexpr int NT = RB * 64, WR = MarchRing<RB>::WR, LR = MarchRing<RB>::LR;
     constexpr int NL = L1<NK, VISC>::N;
@@ -443,9 +486,6 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
     R *sL = sT + (VISC ? WR * MWX : 0);       // NL x LR x MLX
 
     const int pitch = A.pitch;
-    const int i0 = m.xa + (int)blockIdx.x * MTX;
-    const int y0 = m.ya + (int)blockIdx.y * m.H;
-    const int y1 = min(y0 + m.H, m.yb);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ox = i0 - 6, lx0 = i0 - 3;  // window / level-1 column origins
@@ -500,11 +540,17 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
     const Transport<R> tc{R(A.tc.A_mi), R(A.tc.Bt_mi), R(A.tc.Ct_mi), R(A.tc.D_mi), R(A.tc.inv_mi0),
                           R(A.tc.A_k),  R(A.tc.Bt_k),  R(A.tc.Ct_k),  R(A.tc.D_k),  R(A.tc.inv_k0),
                           R(A.tc.Tref)};
-    auto level1_row = [&](int r, const RowCache<R, NK> *rc) {  // wave-uniform row r, column lx0 + lane
+    // metric factors of this lane's level-1 and output columns (fixed for
+    // the whole march)
+    constexpr bool MET = PM != PM_FAST;
+    const R mxl = MET ? R(A.metx[lx0 + lane]) : R(1);
+    const R mxo = MET ? R(A.metx[min(i0 + lane, A.imax - 1)]) : R(1);
+    // wave-uniform row r, column lx0 + lane; my: mety[r]
+    auto level1_row = [&](int r, const RowCache<R, NK> *rc, R my) {
         const int gx = lx0 + lane;
         R out[NL > 0 ? NL : 1];
-        level1_point<S, R, NK, VISC, PERT, FAST>(A, tc, gx, r, ux_at(r), uy_at(gx, r), tx_at(r), ty_at(gx, r),
-                                                  VISC ? tx_at(r)(gx) : R(0), out, rc);
+        level1_point<S, R, NK, VISC, PERT, PM>(A, tc, gx, r, mxl, my, ux_at(r), uy_at(gx, r), tx_at(r),
+                                               ty_at(gx, r), VISC ? tx_at(r)(gx) : R(0), out, rc);
         const int s = (r - ly0) % LR;
 #pragma unroll
         for (int l = 0; l < NL; ++l) sL[(l * LR + s) * MLX + lane] = out[l];
@@ -519,46 +565,45 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
     }
     __syncthreads();
     if constexpr (NL > 0) {
-        for (int rr = wave; rr < 6; rr += RB) level1_row(ly0 + rr, nullptr);
+        for (int rr = wave; rr < 6; rr += RB) level1_row(ly0 + rr, nullptr, MET ? R(A.mety[ly0 + rr]) : R(1));
     }
+    // window rows r0 .. r0+RB-1 into registers: every thread issues its PF
+    // loads (clamped to the last point / row of the grid; the extra values
+    // are never stored), so the count of loads in flight is static
     R pre[PF][NK];
+    auto prefetch = [&](int r0) {
 #pragma unroll
-    for (int p = 0; p < PF; ++p) {
-        const int t = tid + p * NT;
-        if (t < RB * MWX) load_pt(y0 + 6 + t / MWX, t % MWX, pre[p]);
-    }
+        for (int p = 0; p < PF; ++p) {
+            const int t = min(tid + p * NT, RB * MWX - 1);
+            load_pt(min(r0 + t / MWX, A.jmax - 1), t % MWX, pre[p]);
+        }
+    };
+    prefetch(y0 + 6);
 
     for (int base = y0; base < y1; base += RB) {
-        // per-row base data of this step's level-1 and output rows, and W of
-        // this step's outputs, loaded now so that the loads overlap (a) and (b)
-        RowCache<R, NK> rcl, rco;
-        if constexpr (PERT) {
-            load_rows<S, R, NK>(A, base + 3 + wave, rcl);
-            load_rows<S, R, NK>(A, base + wave, rco);
-        }
-        S wpre[NK];
-        {
-            const size_t g = (size_t)(base + wave) * pitch + (i0 + lane);
-#pragma unroll
-            for (int k = 0; k < NK; ++k) wpre[k] = (lane < MTX && !A.first) ? A.W[k][g] : S(0);
-        }
-        // (a) window rows base+6 .. base+RB+5
+        // (a) window rows base+6 .. base+RB+5, loaded during the previous step
 #pragma unroll
         for (int p = 0; p < PF; ++p) {
             const int t = tid + p * NT;
             if (t < RB * MWX) put_pt(base + 6 + t / MWX, t % MWX, pre[p]);
         }
         __syncthreads();
-        if (base + RB < y1) {
-#pragma unroll
-            for (int p = 0; p < PF; ++p) {
-                const int t = tid + p * NT;
-                if (t < RB * MWX) load_pt(base + RB + 6 + t / MWX, t % MWX, pre[p]);
-            }
+        // this step's loads, issued after the wait of (a) so that (a) never
+        // waits for them: per-row data of the level-1 and output rows (base
+        // state of the perturbation form, metric factors), the store data of
+        // the outputs, then the next step's window rows
+        RowCache<R, NK> rcl, rco;
+        if constexpr (PERT) {
+            load_rows<S, R, NK>(A, base + 3 + wave, rcl);
+            load_rows<S, R, NK>(A, base + wave, rco);
         }
+        const R myl = MET ? R(A.mety[base + 3 + wave]) : R(1), myo = MET ? R(A.mety[base + wave]) : R(1);
+        StorePre<S, NK> spre;
+        store_prefetch<S, NK, PERT, PM>(A, min(i0 + lane, A.imax - 1), base + wave, spre);
+        prefetch(base + RB + 6);
         // (b) level-1 row base+3+wave
         if constexpr (NL > 0) {
-            level1_row(base + 3 + wave, PERT ? &rcl : nullptr);
+            level1_row(base + 3 + wave, PERT ? &rcl : nullptr, myl);
             __syncthreads();
         }
         // (c) outputs of row base+wave
@@ -576,10 +621,44 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
             };
             auto lv = [=](int l) { return sL[(l * LR + ls) * MLX + (gx - lx0)]; };
             R qc[NK], uc[NK], f[NK];
-            rhs_point<S, R, NK, VISC, PERT, FAST>(A, gx, gy, ux_at(gy), uy_at(gx, gy), lxa, lya, lv, qc, uc, f,
-                                                  PERT ? &rco : nullptr);
-            stage_store<S, R, NK, false, PERT, FAST>(A, gx, gy, qc, uc, f, wpre);
+            rhs_point<S, R, NK, VISC, PERT, PM>(A, gx, gy, mxo, myo, ux_at(gy), uy_at(gx, gy), lxa, lya, lv, qc,
+                                                uc, f, PERT ? &rco : nullptr);
+            stage_store<S, R, NK, false, PERT, PM>(A, gx, gy, qc, uc, f, &spre);
         }
+    }
+}
+
+// PM_HYBRID: blocks whose outputs lie in the fast zone take PM_FAST, the
+// others PM_MIDU (the same bits, as for every point mode)
+constexpr int PM_HYBRID = 4;
+
+template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
+__global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, MarchMap m)
+{
+    const int i0 = m.xa + (int)blockIdx.x * MTX;
+    if constexpr (PM == PM_HYBRID) {
+        // chunk -1: [ya, y_a) when not empty; 0 .. nmid-1: H rows each; nmid: [y_b, yb)
+        const int nmid = (m.y_b - m.y_a + m.H - 1) / m.H;
+        const int c = (int)blockIdx.y - (m.y_a > m.ya ? 1 : 0);
+        int y0, y1;
+        if (c < 0) {
+            y0 = m.ya;
+            y1 = m.y_a;
+        } else if (c < nmid) {
+            y0 = m.y_a + c * m.H;
+            y1 = min(y0 + m.H, m.y_b);
+        } else {
+            y0 = m.y_b;
+            y1 = m.yb;
+        }
+        if (i0 >= m.fx0 && i0 + MTX <= m.fx1 && y0 >= m.y_a && y1 <= m.y_b)
+            march_body<S, R, NK, VISC, PERT, RB, PM_FAST>(A, i0, y0, y1);
+        else
+            march_body<S, R, NK, VISC, PERT, RB, PM_MIDU>(A, i0, y0, y1);
+    } else {
+        // plain layout (y_a = ya, y_b = yb): chunks of H rows
+        const int y0 = m.ya + (int)blockIdx.y * m.H;
+        march_body<S, R, NK, VISC, PERT, RB, PM>(A, i0, y0, min(y0 + m.H, m.yb));
     }
 }
 
@@ -592,22 +671,28 @@ static size_t stage_smem(int NK, bool visc, int TY)
 }
 
 // The marching region: output columns [xa, xa + 58 n) and rows [ya, yb),
-// every point's 3-point level-1 halo inside the fast zone (interior, outside
-// the buffer/metric layers), rows a multiple of RB; chunk height H from a
+// rows a multiple of RB (march_region, march_hybrid); chunk height H from a
 // simple occupancy model (blocks per wave of the GPU x rows per block).
-template <typename S, int NK, bool VISC, bool PERT, int RB, bool FAST>
+template <typename S, int NK, bool VISC, bool PERT, int RB, int PM>
 static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
 {
-    if (FAST && !A.fast_ok) return false;
-    if (!march_region(FAST, A.D, A.imax, A.jmax, RB, m)) return false;
-    const int rows = m.yb - m.ya;
+    constexpr bool FAST = PM == PM_FAST;
+    if ((FAST || PM == PM_HYBRID) && !A.fast_ok) return false;
+    if constexpr (PM == PM_HYBRID) {
+        if (!march_hybrid(A.D, A.imax, A.jmax, RB, m)) return false;
+    } else {
+        if (!march_region(FAST, A.D, A.imax, A.jmax, RB, m)) return false;
+    }
+    // chunks of H rows over [y_a, y_b); the edge parts are one chunk each
+    const int rows = m.y_b - m.y_a;
+    const int edges = (m.y_a > m.ya ? 1 : 0) + (m.yb > m.y_b ? 1 : 0);
     static int ncu = 0, perblk = 0;
     if (!ncu) {
         int dev = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (ncu <= 0) ncu = 256;
-        auto kern = stage_march_kernel<S, S, NK, VISC, PERT, RB, FAST>;
+        auto kern = stage_march_kernel<S, S, NK, VISC, PERT, RB, PM>;
         const size_t smem = march_smem<S>(NK, VISC, RB);
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perblk, kern, RB * 64, smem) != hipSuccess || perblk < 1)
@@ -617,7 +702,7 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
     long best = -1;
     for (int nc = 1; nc <= rows / (4 * RB); ++nc) {
         const int H = ((rows + nc - 1) / nc + RB - 1) / RB * RB;
-        const int n = (rows + H - 1) / H;
+        const int n = (rows + H - 1) / H + edges;
         const long blocks = (long)m.nstrips * n;
         const long cost = ((blocks + slots - 1) / slots) * (H + 8);
         if (best < 0 || cost < best || (cost == best && n > nchunks)) {
@@ -642,13 +727,29 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     MarchMap m{};
     int nchunks = 0;
     int fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
-    bool inner = false, fastm = false;
+    bool inner = false;
+    int pm = PM_GEN;
     RectSet rs;
-    if constexpr (MARCH) {
-        // the fast zone when it exists (unless GENERAL), else the general region
-        fastm = !GENERAL && A.fast_ok;
-        inner = !RHS_ONLY && (fastm ? march_plan<S, NK, VISC, PERT, RB, true>(A, m, nchunks)
-                                    : march_plan<S, NK, VISC, PERT, RB, false>(A, m, nchunks));
+    if constexpr (MARCH && !RHS_ONLY) {
+        // the fast zone (GENERAL: the general region with its fast blocks)
+        // when there is one, else the general region (interior stencils;
+        // unit spacing drops the 1/h factors)
+        const int gpm = (A.inv_dm == 1.0 && A.inv_dn == 1.0) ? PM_MIDU : PM_MID;
+        pm = gpm;
+        if (A.fast_ok) {
+            if constexpr (GENERAL) {
+                pm = PM_HYBRID;
+                inner = march_plan<S, NK, VISC, PERT, RB, PM_HYBRID>(A, m, nchunks);
+            } else {
+                pm = PM_FAST;
+                inner = march_plan<S, NK, VISC, PERT, RB, PM_FAST>(A, m, nchunks);
+            }
+        }
+        if (!inner) {
+            pm = gpm;
+            inner = pm == PM_MIDU ? march_plan<S, NK, VISC, PERT, RB, PM_MIDU>(A, m, nchunks)
+                                  : march_plan<S, NK, VISC, PERT, RB, PM_MID>(A, m, nchunks);
+        }
         rs = inner ? RectSet::around(A.imax, A.jmax, TY, m.xa, m.xa + m.nstrips * MTX, m.ya, m.yb)
                    : RectSet::around(A.imax, A.jmax, TY, 0, 0, 0, 0);
     } else {
@@ -664,36 +765,48 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     if ((e = hipFuncSetAttribute((const void *)gen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
         hipSuccess)
         return e;
-    if (rs.count() > 0) {
-        hipLaunchKernelGGL(gen, dim3(rs.count()), dim3(NT), smem, (inner && s2) ? s2 : s, A, rs);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    if (!inner) return hipSuccess;
+    // the interior kernel is launched first: the edge tiles, launched after
+    // it (on s2), fill the slots its blocks leave free instead of holding
+    // CUs the long-running interior blocks are waiting for
+    auto edges = [&]() -> hipError_t {
+        if (rs.count() > 0) {
+            hipLaunchKernelGGL(gen, dim3(rs.count()), dim3(NT), smem, (inner && s2) ? s2 : s, A, rs);
+            return hipGetLastError();
+        }
+        return hipSuccess;
+    };
+    if (!inner) return edges();
     if constexpr (!RHS_ONLY && MARCH) {
-        auto kern = fastm ? stage_march_kernel<S, R, NK, VISC, PERT, RB, true>
-                          : stage_march_kernel<S, R, NK, VISC, PERT, RB, false>;
+        void (*kern)(StageArgs<S>, MarchMap) = pm == PM_MIDU ? stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MIDU>
+                                                              : stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MID>;
+        if constexpr (GENERAL) {
+            if (pm == PM_HYBRID) kern = stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_HYBRID>;
+        } else {
+            if (pm == PM_FAST) kern = stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_FAST>;
+        }
         const size_t msm = march_smem<R>(NK, VISC, RB);
         if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)msm)) !=
             hipSuccess)
             return e;
         hipLaunchKernelGGL(kern, dim3(m.nstrips, nchunks), dim3(RB * 64), msm, s, A, m);
-        return hipGetLastError();
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     } else if constexpr (!RHS_ONLY) {
         auto kern = stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>;
         if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
             hipSuccess)
             return e;
         hipLaunchKernelGGL(kern, dim3(fx1 - fx0, fy1 - fy0), dim3(NT), smem, s, A, fx0, fy0);
-        return hipGetLastError();
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    return hipSuccess;
+    return edges();
 }
 
 // variants (same arithmetic):
-//   FP64: 0 -> marching 4 rows/step over the fast zone (the general region
-//         when there is none), edge tiles concurrent on s2;
-//         1 -> marching 4 rows/step over the general region [9, n-9)^2
-//         (buffers and metrics point by point), thin edge band concurrent;
+//   FP64: 0 -> marching 4 rows/step over the general region [9, n-9)^2
+//         (buffers and metrics point by point; blocks inside the fast zone
+//         take the fast code), thin edge band concurrent on s2;
+//         1 -> marching 4 rows/step over the fast zone (the general region
+//         when there is none), the edge ring concurrent;
 //         2 -> as 0 with 8 rows/step
 //   FP32: 0 -> interior tiles (512 threads) over the fast zone, general tiles
 //         elsewhere; 1 -> marching 4 rows/step over the general region;
@@ -706,9 +819,9 @@ static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t
 {
     if constexpr (sizeof(S) == 8) {
         switch (variant) {
-        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true, true>(A, s, s2);
-        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true>(A, s, s2);
-        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true, true>(A, s, s2);
+        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true, true>(A, s, s2);
         }
     } else {
         switch (variant) {

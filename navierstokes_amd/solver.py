@@ -279,6 +279,10 @@ class Solver:
         _lib.check(self.lib.ns_autotune(self.ctx, C.byref(v), _lib.dptr(ms)))
         return v.value, ms
 
+    def set_variant(self, variant: int):
+        """Use stage variant 0..2 (see autotune) without timing."""
+        _lib.check(self.lib.ns_set_variant(self.ctx, int(variant)))
+
     def enable_timing(self, on: bool = True):
         _lib.check(self.lib.ns_enable_timing(self.ctx, int(on)))
 
