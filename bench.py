@@ -65,16 +65,18 @@ def make_case(args):
     return case, desc
 
 
-def profiled_traffic(workload: str, size: int, precision: str):
-    """HBM bytes per stage launch measured with rocprofv3 --pmc (FETCH_SIZE
-    x2 + WRITE_SIZE, calibrated on a plain copy), from the committed summary
-    profiles/hbm_traffic_ns4096.md; None when that configuration has none."""
+def profiled_traffic(workload: str, size: int, precision: str, variant: int):
+    """HBM bytes per stage launch of the given stage variant, measured with
+    rocprofv3 --pmc (FETCH_SIZE x2 + WRITE_SIZE, calibrated on a plain copy),
+    from the committed summary profiles/hbm_traffic_ns4096.md; None when that
+    configuration has none."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_ns4096.md")
     try:
         for line in open(path):
             f = line.split()
-            if len(f) == 5 and f[0] == "traffic" and f[1] == workload and int(f[2]) == size and f[3] == precision:
-                return int(f[4])
+            if (len(f) == 6 and f[0] == "traffic" and f[1] == workload and int(f[2]) == size
+                    and f[3] == precision and f[4] == f"v{variant}"):
+                return int(f[5])
     except OSError:
         pass
     return None
@@ -96,7 +98,9 @@ def cpu_baseline(case, steps: int):
 
 def copy_probe(mib: int):
     """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE
-    of this kernel calibrate the counters against bytes actually moved."""
+    of this kernel calibrate the counters against bytes actually moved, and
+    its rate (read + write bytes over time) is the practical HBM ceiling a
+    streaming kernel is compared with."""
     import torch
     n = mib * 1024 * 1024 // 8
     a = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -104,7 +108,16 @@ def copy_probe(mib: int):
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_MiB": mib, "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_probe_MiB": mib, "bytes_read": n * 8, "bytes_written": n * 8,
+                      "ms": round(ms, 4), "GB_per_s": round(2 * n * 8 / ms / 1e6, 1)}), flush=True)
 
 
 def bench_batch(args):
@@ -425,7 +438,7 @@ def main():
                    "stage_variant_ms": [round(x, 4) for x in tune_ms]},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(args.workload, case.imax, args.precision),
+                     "traffic": profiled_traffic(args.workload, case.imax, args.precision, variant),
                      "kernel": "stage (RHS+LDDRK update+buffer: interior kernel with the edge tiles beside it)",
                      "alg_bytes_per_launch": B_stage * case.npoints,
                      "avg_launch_ms": round(kt["rhs"], 4)},
