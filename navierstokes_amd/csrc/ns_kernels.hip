@@ -766,11 +766,9 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         hipSuccess)
         return e;
     // An edge tile needs most of a CU's LDS, so it cannot share a CU with
-    // interior blocks, and it is latency-bound (a few hundred microseconds).
-    // The thin band of the general-region variant (fewer tiles than CUs) is
-    // launched first, on s2, so that it overlaps the interior kernel from
-    // the start; the thicker ring of the fast-zone variant is launched after
-    // the interior kernel on s2 and fills CUs as they drain.
+    // interior blocks.  The interior kernel is launched first; the edge
+    // tiles, launched after it on s2, take CUs as the interior blocks drain
+    // them instead of holding CUs the long-running interior blocks wait for.
     auto edges = [&](hipStream_t st) -> hipError_t {
         if (rs.count() > 0) {
             hipLaunchKernelGGL(gen, dim3(rs.count()), dim3(NT), smem, st, A, rs);
@@ -779,9 +777,6 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         return hipSuccess;
     };
     if (!inner) return edges(s);
-    if constexpr (MARCH && GENERAL) {
-        if ((e = edges(s2 ? s2 : s)) != hipSuccess) return e;
-    }
     if constexpr (!RHS_ONLY && MARCH) {
         void (*kern)(StageArgs<S>, MarchMap) = pm == PM_MIDU ? stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MIDU>
                                                               : stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MID>;
@@ -796,7 +791,6 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
             return e;
         hipLaunchKernelGGL(kern, dim3(m.nstrips, nchunks), dim3(RB * 64), msm, s, A, m);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if constexpr (GENERAL) return hipSuccess;
     } else if constexpr (!RHS_ONLY) {
         auto kern = stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>;
         if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
@@ -811,7 +805,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
 // variants (same arithmetic):
 //   FP64: 0 -> marching 4 rows/step over the general region [9, n-9)^2
 //         (buffers and metrics point by point; blocks inside the fast zone
-//         take the fast code), the thin edge band launched before it on s2;
+//         take the fast code), the thin edge band concurrent on s2;
 //         1 -> marching 4 rows/step over the fast zone (the general region
 //         when there is none), the edge ring concurrent on s2;
 //         2 -> as 0 with 8 rows/step
