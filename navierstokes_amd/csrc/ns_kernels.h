@@ -164,6 +164,35 @@ __host__ __device__ inline int march_chunks(const MarchMap &m)
     return (m.y_a > m.ya ? 1 : 0) + (m.y_b - m.y_a + m.H - 1) / m.H + (m.yb > m.y_b ? 1 : 0);
 }
 
+// rows [y0, y1) of chunk c (blockIdx.y): [ya, y_a) when not empty, then
+// chunks of H rows over [y_a, y_b), then [y_b, yb) when not empty
+__host__ __device__ inline void march_chunk_rows(const MarchMap &m, int c, int &y0, int &y1)
+{
+    if (m.y_a > m.ya) {
+        if (c == 0) {
+            y0 = m.ya;
+            y1 = m.y_a;
+            return;
+        }
+        --c;
+    }
+    const int nmid = (m.y_b - m.y_a + m.H - 1) / m.H;
+    if (c < nmid) {
+        y0 = m.y_a + c * m.H;
+        y1 = y0 + m.H < m.y_b ? y0 + m.H : m.y_b;
+    } else {
+        y0 = m.y_b;
+        y1 = m.yb;
+    }
+}
+
+// hybrid layout: the block with outputs [i0, i0 + 58) x [y0, y1) lies in the
+// fast zone and takes the fast per-point code
+__host__ __device__ inline bool march_block_fast(const MarchMap &m, int i0, int y0, int y1)
+{
+    return i0 >= m.fx0 && i0 + 58 <= m.fx1 && y0 >= m.y_a && y1 <= m.y_b;
+}
+
 // Output columns / rows of the marching region for strips of 58 and steps of
 // rb rows, rows a multiple of rb.  fast: every output's 3-point level-1 halo
 // inside [max(D,4), n - max(D,4)) in both directions (no buffer, metric or

@@ -637,21 +637,9 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
 {
     const int i0 = m.xa + (int)blockIdx.x * MTX;
     if constexpr (PM == PM_HYBRID) {
-        // chunk -1: [ya, y_a) when not empty; 0 .. nmid-1: H rows each; nmid: [y_b, yb)
-        const int nmid = (m.y_b - m.y_a + m.H - 1) / m.H;
-        const int c = (int)blockIdx.y - (m.y_a > m.ya ? 1 : 0);
         int y0, y1;
-        if (c < 0) {
-            y0 = m.ya;
-            y1 = m.y_a;
-        } else if (c < nmid) {
-            y0 = m.y_a + c * m.H;
-            y1 = min(y0 + m.H, m.y_b);
-        } else {
-            y0 = m.y_b;
-            y1 = m.yb;
-        }
-        if (i0 >= m.fx0 && i0 + MTX <= m.fx1 && y0 >= m.y_a && y1 <= m.y_b)
+        march_chunk_rows(m, (int)blockIdx.y, y0, y1);
+        if (march_block_fast(m, i0, y0, y1))
             march_body<S, R, NK, VISC, PERT, RB, PM_FAST>(A, i0, y0, y1);
         else
             march_body<S, R, NK, VISC, PERT, RB, PM_MIDU>(A, i0, y0, y1);
@@ -684,6 +672,7 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
         if (!march_region(FAST, A.D, A.imax, A.jmax, RB, m)) return false;
     }
     // chunks of H rows over [y_a, y_b); the edge parts are one chunk each
+    // (march_chunks, march_chunk_rows)
     const int rows = m.y_b - m.y_a;
     const int edges = (m.y_a > m.ya ? 1 : 0) + (m.yb > m.y_b ? 1 : 0);
     static int ncu = 0, perblk = 0;

@@ -1,0 +1,86 @@
+// Host-side check of the stage's launch layouts (navierstokes_amd/csrc/ns_kernels.h):
+// for many grids, buffer widths, rows per step and chunk heights, the
+// marching region (fast zone, general region, general region with fast
+// blocks) plus the edge tiles around it (RectSet::around) must cover every
+// grid point exactly once, every marching block must keep its U window and
+// level-1 rows inside the grid and step a whole number of RB rows, and
+// every block that takes the fast per-point code must have its outputs'
+// 3-point level-1 halo inside the fast zone.  Prints "ok <layouts>" or the
+// first failure, and exits non-zero on failure.
+#include <cstdio>
+#include <vector>
+
+#include "ns_kernels.h"
+
+using namespace nsamd;
+
+static int fail(const char *what, int imax, int jmax, int D, int rb, int mode, int H)
+{
+    std::printf("FAIL %s: imax=%d jmax=%d D=%d rb=%d mode=%d H=%d\n", what, imax, jmax, D, rb, mode, H);
+    return 1;
+}
+
+// mode 0: fast zone; 1: general region; 2: general region with fast blocks
+static int check(int imax, int jmax, int D, int rb, int mode, int H, long &layouts)
+{
+    MarchMap m{};
+    const bool ok = mode == 0 ? march_region(true, D, imax, jmax, rb, m)
+                    : mode == 1 ? march_region(false, D, imax, jmax, rb, m)
+                                : march_hybrid(D, imax, jmax, rb, m);
+    if (!ok) return 0;
+    if ((m.y_b - m.y_a) < rb || H % rb) return 0;
+    m.H = H;
+    ++layouts;
+    const int Dm = D > 4 ? D : 4;
+    std::vector<int> cnt((size_t)imax * jmax, 0);
+    const int nch = march_chunks(m);
+    for (int s = 0; s < m.nstrips; ++s)
+        for (int c = 0; c < nch; ++c) {
+            const int i0 = m.xa + 58 * s;
+            int y0, y1;
+            march_chunk_rows(m, c, y0, y1);
+            if (y1 <= y0 || (y1 - y0) % rb) return fail("chunk rows", imax, jmax, D, rb, mode, H);
+            // window rows y0-6 .. y1+5 (level-1 rows y0-3 .. y1+2), columns i0-6 .. i0+63
+            if (y0 - 6 < 0 || y1 + 5 > jmax - 1 || i0 - 6 < 0 || i0 + 63 > imax - 1)
+                return fail("window outside the grid", imax, jmax, D, rb, mode, H);
+            // every derivative of the interior-stencil code is the interior one
+            if (y0 - 3 < 3 || y1 + 2 > jmax - 4 || i0 - 3 < 3 || i0 + 60 > imax - 4)
+                return fail("one-sided stencil in a marching block", imax, jmax, D, rb, mode, H);
+            const bool fast = mode == 0 || (mode == 2 && march_block_fast(m, i0, y0, y1));
+            if (fast && (i0 - 3 < Dm || i0 + 58 + 3 > imax - Dm || y0 - 3 < Dm || y1 + 3 > jmax - Dm))
+                return fail("fast block outside the fast zone", imax, jmax, D, rb, mode, H);
+            for (int j = y0; j < y1; ++j)
+                for (int i = i0; i < i0 + 58; ++i) ++cnt[(size_t)j * imax + i];
+        }
+    for (int ty : {8, 16}) {
+        std::vector<int> c2 = cnt;
+        const RectSet rs = RectSet::around(imax, jmax, ty, m.xa, m.xa + 58 * m.nstrips, m.ya, m.yb);
+        for (int t = 0; t < rs.count(); ++t) {
+            int i0, i1, j0, j1;
+            rs.tile(t, i0, i1, j0, j1);
+            if (i0 < 0 || j0 < 0 || i1 > imax || j1 > jmax || i1 - i0 > 64 || j1 - j0 > ty || i1 <= i0 || j1 <= j0)
+                return fail("edge tile", imax, jmax, D, rb, mode, H);
+            for (int j = j0; j < j1; ++j)
+                for (int i = i0; i < i1; ++i) ++c2[(size_t)j * imax + i];
+        }
+        for (int v : c2)
+            if (v != 1) return fail("coverage", imax, jmax, D, rb, mode, H);
+    }
+    return 0;
+}
+
+int main()
+{
+    long layouts = 0;
+    const int sizes[] = {40, 77, 128, 150, 201, 256, 301, 330, 512, 700, 1031};
+    const int Ds[] = {4, 5, 6, 8, 12, 20, 30, 40, 60};
+    for (int imax : sizes)
+        for (int jmax : sizes)
+            for (int D : Ds)
+                for (int rb : {4, 8})
+                    for (int mode = 0; mode < 3; ++mode)
+                        for (int H : {rb, 4 * rb, 7 * rb, 16 * rb, 50 * rb, 4096})
+                            if (check(imax, jmax, D, rb, mode, H - H % rb, layouts)) return 1;
+    std::printf("ok %ld\n", layouts);
+    return 0;
+}
