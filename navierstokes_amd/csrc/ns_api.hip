@@ -521,6 +521,19 @@ int download_aos(ns_ctx *c, const double *src, double *U, int nk)
 
 int set_ptr_table(ns_ctx *c, int slot, void *buf, int nk, size_t esz = sizeof(double));
 
+// One fork/join round trip between two new streams, synchronised: both
+// streams are in use before any step is captured into a graph, so the
+// side stream's first operation is never one recorded inside a capture.
+hipError_t touch_streams(hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join)
+{
+    hipError_t e = hipEventRecord(fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+    if (e == hipSuccess) e = hipEventRecord(join, side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
 // state <-> host AoS; the perturbation form stores q = U - Ub(k,j) in FP32
 int upload_state(ns_ctx *c, const double *U)
 {
@@ -757,6 +770,7 @@ int32_t ns_create(const ns_params *p, ns_ctx **out)
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = touch_streams(c->stream, c->side, c->fork, c->join);
     if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "side stream/events: %s", hipGetErrorString(e)));
     const size_t sbytes = (size_t)P.nk * c->plane * c->esz;
     for (void **pp : {&c->S[0], &c->S[1], &c->S[2], &c->Wr}) {
@@ -824,6 +838,7 @@ int32_t ns_set_stream(ns_ctx *c, void *s)
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->own_stream = true;
     }
+    HIPCHK(touch_streams(c->stream, c->side, c->fork, c->join));
     return NS_OK;
 }
 
@@ -1448,6 +1463,7 @@ int32_t ns_batch_create(const ns_params *params, int32_t ncases, ns_batch **out)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = touch_streams(b->stream, b->side, b->fork, b->join);
     if (e != hipSuccess) {
         batch_release(b);
         delete b;
