@@ -410,15 +410,18 @@ def main():
     # per-kernel device time (HIP events on the library's stream)
     s.enable_timing(True)
     s.run(args.warmup + args.steps, args.timing_steps)
-    kt = {name: s.kernel_time(name)[0] for name in ("rhs", "filter_x", "filter_y", "bc")}
+    kt = {name: s.kernel_time(name)[0] for name in ("rhs", "interior", "filter_x", "filter_y", "bc")}
+    npts_int = s.stage_interior_points()
     s.enable_timing(False)
     s.close()
 
     nk, elem = case.nk, (4 if prec else 8)
     B_step = alg_bytes_per_point_step(nk, elem)
     B_stage = B_step / 6.0  # per point per stage-kernel launch (average over the 6 stages)
-    t_stage = kt["rhs"] * 1e-3
-    ach = B_stage * case.npoints / t_stage / 1e9
+    # the dominant kernel: the stage's interior kernel, timed with HIP events
+    # on its own stream, charged the algorithmic bytes of the points it covers
+    t_int = kt["interior"] * 1e-3
+    ach = B_stage * npts_int / t_int / 1e9
     step_ach = B_step * case.npoints / (ms_step * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -439,9 +442,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(args.workload, case.imax, args.precision, variant),
-                     "kernel": "stage (RHS+LDDRK update+buffer: interior kernel with the edge tiles beside it)",
-                     "alg_bytes_per_launch": B_stage * case.npoints,
-                     "avg_launch_ms": round(kt["rhs"], 4)},
+                     "kernel": "stage interior kernel (RHS + LDDRK update + buffer over the interior region; "
+                               "the edge tiles run beside it on a second stream)",
+                     "interior_points": npts_int,
+                     "alg_bytes_per_launch": B_stage * npts_int,
+                     "avg_launch_ms": round(kt["interior"], 4)},
         "step_roofline": {"alg_bytes_per_point_step": B_step, "achieved_GBs": round(step_ach, 1),
                           "frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "kernel_ms": {k: round(v, 4) for k, v in kt.items()},

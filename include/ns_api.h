@@ -222,9 +222,15 @@ int32_t ns_set_variant(ns_ctx *ctx, int32_t variant);
 
 /* measurement ----------------------------------------------------------------- */
 /* average device time (ms) of each kernel class over the last ns_rk_steps
-   call with timing enabled: names "rhs", "filter_x", "filter_y", "bc" */
+   call with timing enabled: names "rhs" (the whole stage launch: interior
+   kernel and edge tiles, fork to join), "interior" (the stage's interior
+   kernel alone, events on its own stream), "filter_x", "filter_y", "bc" */
 int32_t ns_enable_timing(ns_ctx *ctx, int32_t on);
 int32_t ns_kernel_time(ns_ctx *ctx, const char *name, double *avg_ms, int32_t *count);
+/* output points covered by the stage's interior kernel in the last timed
+   stage (the rest of the grid runs as edge tiles).
+   No reference counterpart: measurement only. */
+int32_t ns_stage_interior(ns_ctx *ctx, int64_t *points);
 
 #ifdef __cplusplus
 }

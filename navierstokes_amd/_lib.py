@@ -84,6 +84,7 @@ SIGNATURES = {
     "ns_acousticfont": (C.c_int32, [_V, C.c_double, C.c_int32, _DP]),
     "ns_autotune": (C.c_int32, [_V, _IP, _DP]),
     "ns_set_variant": (C.c_int32, [_V, C.c_int32]),
+    "ns_stage_interior": (C.c_int32, [_V, C.POINTER(C.c_int64)]),
     "ns_enable_timing": (C.c_int32, [_V, C.c_int32]),
     "ns_kernel_time": (C.c_int32, [_V, C.c_char_p, _DP, _IP]),
     "ns_batch_create": (C.c_int32, [C.POINTER(NSParams), C.c_int32, C.POINTER(_V)]),

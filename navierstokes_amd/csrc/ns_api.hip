@@ -22,9 +22,9 @@
 
 namespace nsamd {
 hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                        hipStream_t s2);
+                        hipStream_t s2, StageTiming *tm = nullptr);
 hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                        hipStream_t s2);
+                        hipStream_t s2, StageTiming *tm = nullptr);
 hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s);
 hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s);
@@ -101,8 +101,11 @@ int fail(int code, const char *fmt, ...)
         if (e_ != hipSuccess) return fail(NS_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-enum TimerClass { TC_STAGE = 0, TC_FX, TC_FY, TC_BC, TC_N };
-const char *TC_NAMES[TC_N] = {"rhs", "filter_x", "filter_y", "bc"};
+// "rhs": the whole stage launch (interior kernel, edge tiles beside it on
+// the side stream, fork to join); "interior": the interior kernel alone,
+// timed on its own stream
+enum TimerClass { TC_STAGE = 0, TC_FX, TC_FY, TC_BC, TC_INT, TC_N };
+const char *TC_NAMES[TC_N] = {"rhs", "filter_x", "filter_y", "bc", "interior"};
 
 }  // namespace
 
@@ -138,6 +141,7 @@ struct ns_ctx {
     hipEvent_t ev[2 * 24] = {};
     double tsum[TC_N] = {};
     long tcount[TC_N] = {};
+    long long interior_points = 0;  // output points of the interior kernel in the last timed stage
     // reverse mode: observer arc, checkpoints and adjoint work planes
     struct Adj {
         int2 *arc = nullptr;
@@ -362,7 +366,13 @@ int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed)
     // stream (fork after the previous work, join before the filters)
     HIPCHK(hipEventRecord(c->fork, c->stream));
     HIPCHK(hipStreamWaitEvent(c->side, c->fork, 0));
-    HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream, c->side));
+    StageTiming tm;
+    if (timed) {
+        tm.before = c->ev[5];
+        tm.after = c->ev[6];
+    }
+    HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream, c->side,
+                        timed ? &tm : nullptr));
     HIPCHK(hipEventRecord(c->join, c->side));
     HIPCHK(hipStreamWaitEvent(c->stream, c->join, 0));
     tick(1);
@@ -374,12 +384,19 @@ int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed)
     tick(4);
     if (timed) {
         HIPCHK(hipEventSynchronize(c->ev[4]));
-        for (int t = 0; t < TC_N; ++t) {
+        for (int t = 0; t < TC_INT; ++t) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, c->ev[t], c->ev[t + 1]));
             c->tsum[t] += ms;
             c->tcount[t] += 1;
         }
+        if (tm.interior_points > 0) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
+            c->tsum[TC_INT] += ms;
+            c->tcount[TC_INT] += 1;
+        }
+        c->interior_points = tm.interior_points;
     }
     return NS_OK;
 }
@@ -438,7 +455,9 @@ int autotune_stage_t(ns_ctx *c)
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
         c->tune_ms[v] = ms / reps;
-        if (ms / reps < best_ms) {
+        // a later variant must be more than 3 % faster to replace an earlier
+        // one, so that variants within timing noise do not flip from run to run
+        if (ms / reps < best_ms * (best_ms < 1e299 ? 0.97 : 1.0)) {
             best_ms = ms / reps;
             best = v;
         }
@@ -1243,6 +1262,13 @@ int32_t ns_enable_timing(ns_ctx *c, int32_t on)
         c->tsum[t] = 0.0;
         c->tcount[t] = 0;
     }
+    return NS_OK;
+}
+
+int32_t ns_stage_interior(ns_ctx *c, int64_t *points)
+{
+    if (!c || !points) return fail(NS_ERR_PARAM, "null argument");
+    *points = c->interior_points;
     return NS_OK;
 }
 

@@ -153,6 +153,14 @@ struct RectSet {
 // H rows over [y_a, y_b), one chunk [y_b, yb) (if not empty).  Hybrid layout
 // (fx1 > fx0): blocks whose outputs lie in [fx0, fx1) x [y_a, y_b) take the
 // fast per-point code.
+// Optional timing hooks of a stage launch: events recorded on the interior
+// kernel's stream right before and after its launch, and the number of
+// output points that kernel covers.
+struct StageTiming {
+    hipEvent_t before = nullptr, after = nullptr;
+    long long interior_points = 0;
+};
+
 struct MarchMap {
     int xa, nstrips, ya, yb, H;
     int y_a, y_b;

@@ -709,7 +709,7 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
 // the two run concurrently; the caller orders s2 after s and joins it back).
 // The RHS-only mode runs the general tiles everywhere.
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, int RB, bool MARCH, bool GENERAL = false>
-static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream_t s2)
+static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream_t s2, StageTiming *tm)
 {
     using R = S;
     constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
@@ -765,7 +765,14 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         }
         return hipSuccess;
     };
+    if (tm) {
+        if constexpr (MARCH && !RHS_ONLY)
+            tm->interior_points = inner ? (long long)m.nstrips * MTX * (m.yb - m.ya) : 0;
+        else
+            tm->interior_points = inner ? (long long)(fx1 - fx0) * STX * (fy1 - fy0) * TY : 0;
+    }
     if (!inner) return edges(s);
+    if (tm && tm->before && (e = hipEventRecord(tm->before, s)) != hipSuccess) return e;
     if constexpr (!RHS_ONLY && MARCH) {
         void (*kern)(StageArgs<S>, MarchMap) = pm == PM_MIDU ? stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MIDU>
                                                               : stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MID>;
@@ -780,6 +787,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
             return e;
         hipLaunchKernelGGL(kern, dim3(m.nstrips, nchunks), dim3(RB * 64), msm, s, A, m);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (tm && tm->after && (e = hipEventRecord(tm->after, s)) != hipSuccess) return e;
     } else if constexpr (!RHS_ONLY) {
         auto kern = stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>;
         if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
@@ -787,6 +795,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
             return e;
         hipLaunchKernelGGL(kern, dim3(fx1 - fx0, fy1 - fy0), dim3(NT), smem, s, A, fx0, fy0);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (tm && tm->after && (e = hipEventRecord(tm->after, s)) != hipSuccess) return e;
     }
     return edges(s2 ? s2 : s);
 }
@@ -805,39 +814,39 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
 // code at unit metric and spacing performs the same operations); in FP32
 // (contraction allowed) the tile and marching kernels may round differently.
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
-static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s, hipStream_t s2)
+static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s, hipStream_t s2, StageTiming *tm)
 {
     if constexpr (sizeof(S) == 8) {
         switch (variant) {
-        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
-        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true, true>(A, s, s2);
-        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true, true>(A, s, s2);
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2, tm);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 8, true, true>(A, s, s2, tm);
+        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true, true>(A, s, s2, tm);
         }
     } else {
         switch (variant) {
-        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true, true>(A, s, s2);
-        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2);
-        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, false>(A, s, s2);
+        case 1: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true, true>(A, s, s2, tm);
+        case 2: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, true>(A, s, s2, tm);
+        default: return launch_stage_t<S, NK, VISC, RHS_ONLY, PERT, 512, 4, false>(A, s, s2, tm);
         }
     }
 }
 
 template <typename S, bool PERT>
 static hipError_t launch_stage_p(const StageArgs<S> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                                 hipStream_t s2)
+                                 hipStream_t s2, StageTiming *tm)
 {
     if (nk == 4) {
         if (visc)
-            return rhs_only ? launch_stage_v<S, 4, true, true, PERT>(A, variant, s, s2)
-                            : launch_stage_v<S, 4, true, false, PERT>(A, variant, s, s2);
-        return rhs_only ? launch_stage_v<S, 4, false, true, PERT>(A, variant, s, s2)
-                        : launch_stage_v<S, 4, false, false, PERT>(A, variant, s, s2);
+            return rhs_only ? launch_stage_v<S, 4, true, true, PERT>(A, variant, s, s2, tm)
+                            : launch_stage_v<S, 4, true, false, PERT>(A, variant, s, s2, tm);
+        return rhs_only ? launch_stage_v<S, 4, false, true, PERT>(A, variant, s, s2, tm)
+                        : launch_stage_v<S, 4, false, false, PERT>(A, variant, s, s2, tm);
     }
     if (visc)
-        return rhs_only ? launch_stage_v<S, 5, true, true, PERT>(A, variant, s, s2)
-                        : launch_stage_v<S, 5, true, false, PERT>(A, variant, s, s2);
-    return rhs_only ? launch_stage_v<S, 5, false, true, PERT>(A, variant, s, s2)
-                    : launch_stage_v<S, 5, false, false, PERT>(A, variant, s, s2);
+        return rhs_only ? launch_stage_v<S, 5, true, true, PERT>(A, variant, s, s2, tm)
+                        : launch_stage_v<S, 5, true, false, PERT>(A, variant, s, s2, tm);
+    return rhs_only ? launch_stage_v<S, 5, false, true, PERT>(A, variant, s, s2, tm)
+                    : launch_stage_v<S, 5, false, false, PERT>(A, variant, s, s2, tm);
 }
 
 template <typename S, int NK, bool VISC, bool PERT, int NT, bool FAST>
@@ -907,17 +916,17 @@ hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, 
 
 #if defined(NS_KERNELS_F64)
 hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                        hipStream_t s2)
+                        hipStream_t s2, StageTiming *tm)
 {
-    return launch_stage_p<double, false>(A, nk, visc, rhs_only, variant, s, s2);
+    return launch_stage_p<double, false>(A, nk, visc, rhs_only, variant, s, s2, tm);
 }
 #endif
 
 #if defined(NS_KERNELS_F32)
 hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                        hipStream_t s2)
+                        hipStream_t s2, StageTiming *tm)
 {
-    return launch_stage_p<float, true>(A, nk, visc, rhs_only, variant, s, s2);
+    return launch_stage_p<float, true>(A, nk, visc, rhs_only, variant, s, s2, tm);
 }
 #endif
 

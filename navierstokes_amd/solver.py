@@ -286,6 +286,12 @@ class Solver:
     def enable_timing(self, on: bool = True):
         _lib.check(self.lib.ns_enable_timing(self.ctx, int(on)))
 
+    def stage_interior_points(self) -> int:
+        """Output points of the stage's interior kernel in the last timed stage."""
+        n = C.c_int64()
+        _lib.check(self.lib.ns_stage_interior(self.ctx, C.byref(n)))
+        return n.value
+
     def kernel_time(self, name: str):
         avg = C.c_double()
         cnt = C.c_int32()
