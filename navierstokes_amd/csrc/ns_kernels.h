@@ -103,14 +103,16 @@ __host__ __device__ inline void fast_range(int fast_ok, int D, int n, int T, int
     if (!fast_ok || n - Dm - 3 - T < 0 || b1 < b0) b0 = b1 = 0;
 }
 
-// Up to four rectangles [x0,x1) x [y0,y1), each cut into 64 x ty tiles from
+// Up to four rectangles [x0,x1) x [y0,y1), each cut into tw x ty tiles from
 // its own origin: the grid around an inner rectangle (bottom band, top band,
 // left and right strips), or the whole grid when the inner one is empty.
+// `which` selects rectangles (bit 0 bottom, 1 top, 2 left, 3 right), so the
+// bands and the side strips can use differently shaped tiles.
 struct RectSet {
     int x0[4], x1[4], y0[4], y1[4];
     int tx[4];      // tiles across each rectangle
     int start[5];   // first tile of each rectangle (start[4] = total)
-    int ty;
+    int tw, ty;     // tile width and height
     __host__ __device__ int count() const { return start[4]; }
     __host__ __device__ void tile(int t, int &i0, int &i1, int &j0, int &j1) const
     {
@@ -118,14 +120,15 @@ struct RectSet {
         while (r < 3 && t >= start[r + 1]) ++r;
         t -= start[r];
         const int by = t / tx[r], bx = t - by * tx[r];
-        i0 = x0[r] + bx * 64;
-        i1 = i0 + 64 < x1[r] ? i0 + 64 : x1[r];
+        i0 = x0[r] + bx * tw;
+        i1 = i0 + tw < x1[r] ? i0 + tw : x1[r];
         j0 = y0[r] + by * ty;
         j1 = j0 + ty < y1[r] ? j0 + ty : y1[r];
     }
-    static RectSet around(int imax, int jmax, int ty, int xa, int xb, int ya, int yb)
+    static RectSet around(int imax, int jmax, int tw, int ty, int xa, int xb, int ya, int yb, int which = 15)
     {
         RectSet s{};
+        s.tw = tw;
         s.ty = ty;
         if (xb <= xa || yb <= ya) xa = xb = ya = yb = 0;
         const int r[4][4] = {{0, imax, 0, ya}, {0, imax, yb, jmax}, {0, xa, ya, yb}, {xb, imax, ya, yb}};
@@ -137,8 +140,8 @@ struct RectSet {
             s.x1[q] = X1;
             s.y0[q] = Y0;
             s.y1[q] = Y1;
-            const bool empty = X1 <= X0 || Y1 <= Y0 || (q > 0 && xa == xb);
-            s.tx[q] = empty ? 1 : (X1 - X0 + 63) / 64;
+            const bool empty = X1 <= X0 || Y1 <= Y0 || (q > 0 && xa == xb) || !((which >> q) & 1);
+            s.tx[q] = empty ? 1 : (X1 - X0 + tw - 1) / tw;
             s.start[q] = n;
             if (!empty) n += s.tx[q] * ((Y1 - Y0 + ty - 1) / ty);
         }
@@ -147,12 +150,6 @@ struct RectSet {
     }
 };
 
-// marching region: output columns [xa, xa + 58 nstrips), rows [ya, yb) in
-// chunks of H rows (blockIdx.x = strip, blockIdx.y = chunk)
-// Rows [ya, yb) are marched as: one chunk [ya, y_a) (if not empty), chunks of
-// H rows over [y_a, y_b), one chunk [y_b, yb) (if not empty).  Hybrid layout
-// (fx1 > fx0): blocks whose outputs lie in [fx0, fx1) x [y_a, y_b) take the
-// fast per-point code.
 // Optional timing hooks of a stage launch: events recorded on the interior
 // kernel's stream right before and after its launch, and the number of
 // output points that kernel covers.

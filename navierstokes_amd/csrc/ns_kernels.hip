@@ -301,13 +301,13 @@ __device__ __forceinline__ void stage_store(const StageArgs<S> &A, int gx, int g
 
 // ---------------------------------------------------------------------------
 // Tile version: one block computes the outputs [i0,i1) x [j0,j1) (at most
-// STX x TY) from an LDS box holding U (and T) on the tile plus a 6-point halo
+// TX x TY) from an LDS box holding U (and T) on the tile plus a 6-point halo
 // and the level-1 fields on the tile plus a 3-point halo.
 // ---------------------------------------------------------------------------
-template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT, int PM>
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT, int PM, int TX = STX>
 __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1, int j0, int j1)
 {
-    constexpr int TX = STX;
+    static_assert(NT % TX == 0, "the block covers whole tile rows");
     constexpr int WX = TX + 12, WY = TY + 12;
     constexpr int LX = TX + 6, LY = TY + 6;
     constexpr int NL = L1<NK, VISC>::N;
@@ -384,10 +384,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1
     }
 
     // phase 3: RHS at the tile's points, then the stage update
-    const int gx = i0 + (tid & 63);
+    const int gx = i0 + tid % TX;
     if (gx >= i1) return;
     const int lx0 = rx.l0, ly0 = ry.l0;
-    for (int gy = j0 + (tid >> 6); gy < j1; gy += NT / 64) {
+    for (int gy = j0 + tid / TX; gy < j1; gy += NT / TX) {
         auto lxa = [=](int l) {
             const R *row = sL + (l * LY + (gy - ly0)) * LX;
             return [row, lx0](int q) { return row[q - lx0]; };
@@ -413,13 +413,26 @@ __global__ __launch_bounds__(NT) void stage_kernel_fast(StageArgs<S> A, int bx0,
     stage_tile<S, R, NK, VISC, TY, false, PERT, NT, PM_FAST>(A, i0, i0 + STX, j0, j0 + TY);
 }
 
-// general tiles over up to four rectangles (the grid around the interior region)
+// Edge tiles around the interior region in one launch: the first
+// sides.count() blocks take SSX x 4 TY tiles over the side strips beside it
+// (a few columns wide: wide tiles there would be mostly empty and 4x as
+// many), the rest STX x TY tiles over the bands below and above it (the
+// whole grid when there is no interior region).  Both shapes use the same
+// LDS box, transposed.  One launch, so that band tiles need not wait for the
+// last side tile as a second launch on the same stream would.
+constexpr int SSX = 16;
 template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT>
-__global__ __launch_bounds__(NT) void stage_kernel_rects(StageArgs<S> A, RectSet rs)
+__global__ __launch_bounds__(NT) void stage_kernel_edges(StageArgs<S> A, RectSet sides, RectSet bands)
 {
     int i0, i1, j0, j1;
-    rs.tile(blockIdx.x, i0, i1, j0, j1);
-    stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, PM_GEN>(A, i0, i1, j0, j1);
+    const int t = (int)blockIdx.x;
+    if (t < sides.count()) {
+        sides.tile(t, i0, i1, j0, j1);
+        stage_tile<S, R, NK, VISC, 4 * TY, RHS_ONLY, PERT, NT, PM_GEN, SSX>(A, i0, i1, j0, j1);
+    } else {
+        bands.tile(t - sides.count(), i0, i1, j0, j1);
+        stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, PM_GEN>(A, i0, i1, j0, j1);
+    }
 }
 
 // Batched launch: one block per entry of a tile table spanning every case;
@@ -651,9 +664,9 @@ __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, Ma
 }
 
 template <typename R>
-static size_t stage_smem(int NK, bool visc, int TY)
+static size_t stage_smem(int NK, bool visc, int TY, int TX = STX)
 {
-    const int WX = STX + 12, WY = TY + 12, LX = STX + 6, LY = TY + 6;
+    const int WX = TX + 12, WY = TY + 12, LX = TX + 6, LY = TY + 6;
     const int NL = (visc ? 5 : 0) + (NK >= 5 ? 1 : 0);
     return sizeof(R) * ((size_t)NK * WY * WX + (visc ? (size_t)WY * WX : 0) + (size_t)NL * LY * LX);
 }
@@ -718,7 +731,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     int fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
     bool inner = false;
     int pm = PM_GEN;
-    RectSet rs;
+    int xa = 0, xb = 0, ya = 0, yb = 0;  // interior region; the edge tiles cover the rest
     if constexpr (MARCH && !RHS_ONLY) {
         // the fast zone (GENERAL: the general region with its fast blocks)
         // when there is one, else the general region (interior stencils;
@@ -739,19 +752,22 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
             inner = pm == PM_MIDU ? march_plan<S, NK, VISC, PERT, RB, PM_MIDU>(A, m, nchunks)
                                   : march_plan<S, NK, VISC, PERT, RB, PM_MID>(A, m, nchunks);
         }
-        rs = inner ? RectSet::around(A.imax, A.jmax, TY, m.xa, m.xa + m.nstrips * MTX, m.ya, m.yb)
-                   : RectSet::around(A.imax, A.jmax, TY, 0, 0, 0, 0);
+        if (inner) xa = m.xa, xb = m.xa + m.nstrips * MTX, ya = m.ya, yb = m.yb;
     } else {
         fast_range(RHS_ONLY ? 0 : A.fast_ok, A.D, A.imax, STX, fx0, fx1);
         fast_range(RHS_ONLY ? 0 : A.fast_ok, A.D, A.jmax, TY, fy0, fy1);
         inner = fx1 > fx0 && fy1 > fy0;
-        rs = inner ? RectSet::around(A.imax, A.jmax, TY, fx0 * STX, fx1 * STX, fy0 * TY, fy1 * TY)
-                   : RectSet::around(A.imax, A.jmax, TY, 0, 0, 0, 0);
+        if (inner) xa = fx0 * STX, xb = fx1 * STX, ya = fy0 * TY, yb = fy1 * TY;
     }
     hipError_t e;
+    // edge tiles (stage_kernel_edges): side strips, then the bands
+    const RectSet sides = RectSet::around(A.imax, A.jmax, SSX, 4 * TY, xa, xb, ya, yb, inner ? 12 : 0);
+    const RectSet bands = RectSet::around(A.imax, A.jmax, STX, TY, xa, xb, ya, yb, inner ? 3 : 15);
     const size_t smem = stage_smem<R>(NK, VISC, TY);
-    auto gen = stage_kernel_rects<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT>;
-    if ((e = hipFuncSetAttribute((const void *)gen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
+    const size_t ssm = stage_smem<R>(NK, VISC, 4 * TY, SSX);
+    const size_t esm = ssm > smem ? ssm : smem;
+    auto gen = stage_kernel_edges<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT>;
+    if ((e = hipFuncSetAttribute((const void *)gen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)esm)) !=
         hipSuccess)
         return e;
     // An edge tile needs most of a CU's LDS, so it cannot share a CU with
@@ -759,8 +775,9 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     // tiles, launched after it on s2, take CUs as the interior blocks drain
     // them instead of holding CUs the long-running interior blocks wait for.
     auto edges = [&](hipStream_t st) -> hipError_t {
-        if (rs.count() > 0) {
-            hipLaunchKernelGGL(gen, dim3(rs.count()), dim3(NT), smem, st, A, rs);
+        const int n = sides.count() + bands.count();
+        if (n > 0) {
+            hipLaunchKernelGGL(gen, dim3(n), dim3(NT), esm, st, A, sides, bands);
             return hipGetLastError();
         }
         return hipSuccess;

@@ -1,8 +1,9 @@
 // Host-side check of the stage's launch layouts (navierstokes_amd/csrc/ns_kernels.h):
 // for many grids, buffer widths, rows per step and chunk heights, the
 // marching region (fast zone, general region, general region with fast
-// blocks) plus the edge tiles around it (RectSet::around) must cover every
-// grid point exactly once, every marching block must keep its U window and
+// blocks) plus the edge tiles around it (RectSet::around: wide tiles over the
+// bands below and above, narrow tall tiles over the side strips) must cover
+// every grid point exactly once, every marching block must keep its U window and
 // level-1 rows inside the grid and step a whole number of RB rows, and
 // every block that takes the fast per-point code must have its outputs'
 // 3-point level-1 halo inside the fast zone.  Prints "ok <layouts>" or the
@@ -54,18 +55,42 @@ static int check(int imax, int jmax, int D, int rb, int mode, int H, long &layou
         }
     for (int ty : {8, 16}) {
         std::vector<int> c2 = cnt;
-        const RectSet rs = RectSet::around(imax, jmax, ty, m.xa, m.xa + 58 * m.nstrips, m.ya, m.yb);
-        for (int t = 0; t < rs.count(); ++t) {
-            int i0, i1, j0, j1;
-            rs.tile(t, i0, i1, j0, j1);
-            if (i0 < 0 || j0 < 0 || i1 > imax || j1 > jmax || i1 - i0 > 64 || j1 - j0 > ty || i1 <= i0 || j1 <= j0)
-                return fail("edge tile", imax, jmax, D, rb, mode, H);
-            for (int j = j0; j < j1; ++j)
-                for (int i = i0; i < i1; ++i) ++c2[(size_t)j * imax + i];
-        }
+        // bands below and above the marching region in 64 x ty tiles, side
+        // strips beside it in 16 x 4ty tiles (the stage launch's edge sets)
+        const int xa = m.xa, xb = m.xa + 58 * m.nstrips;
+        const RectSet sets[2] = {RectSet::around(imax, jmax, 64, ty, xa, xb, m.ya, m.yb, 3),
+                                 RectSet::around(imax, jmax, 16, 4 * ty, xa, xb, m.ya, m.yb, 12)};
+        for (const RectSet &rs : sets)
+            for (int t = 0; t < rs.count(); ++t) {
+                int i0, i1, j0, j1;
+                rs.tile(t, i0, i1, j0, j1);
+                if (i0 < 0 || j0 < 0 || i1 > imax || j1 > jmax || i1 - i0 > rs.tw || j1 - j0 > rs.ty || i1 <= i0 ||
+                    j1 <= j0)
+                    return fail("edge tile", imax, jmax, D, rb, mode, H);
+                for (int j = j0; j < j1; ++j)
+                    for (int i = i0; i < i1; ++i) ++c2[(size_t)j * imax + i];
+            }
         for (int v : c2)
             if (v != 1) return fail("coverage", imax, jmax, D, rb, mode, H);
     }
+    return 0;
+}
+
+// no interior region: the bands' set alone tiles the whole grid, the side set is empty
+static int check_whole(int imax, int jmax, int ty)
+{
+    const RectSet all = RectSet::around(imax, jmax, 64, ty, 0, 0, 0, 0, 15);
+    const RectSet none = RectSet::around(imax, jmax, 16, 4 * ty, 0, 0, 0, 0, 0);
+    if (none.count() != 0) return fail("side tiles without an interior", imax, jmax, 0, 0, -1, 0);
+    std::vector<int> c((size_t)imax * jmax, 0);
+    for (int t = 0; t < all.count(); ++t) {
+        int i0, i1, j0, j1;
+        all.tile(t, i0, i1, j0, j1);
+        for (int j = j0; j < j1; ++j)
+            for (int i = i0; i < i1; ++i) ++c[(size_t)j * imax + i];
+    }
+    for (int v : c)
+        if (v != 1) return fail("whole-grid coverage", imax, jmax, 0, 0, -1, 0);
     return 0;
 }
 
@@ -81,6 +106,10 @@ int main()
                     for (int mode = 0; mode < 3; ++mode)
                         for (int H : {rb, 4 * rb, 7 * rb, 16 * rb, 50 * rb, 4096})
                             if (check(imax, jmax, D, rb, mode, H - H % rb, layouts)) return 1;
+    for (int imax : sizes)
+        for (int jmax : sizes)
+            for (int ty : {8, 16})
+                if (check_whole(imax, jmax, ty)) return 1;
     std::printf("ok %ld\n", layouts);
     return 0;
 }
