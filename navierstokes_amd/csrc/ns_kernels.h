@@ -201,13 +201,16 @@ __host__ __device__ inline bool march_block_fast(const MarchMap &m, int i0, int 
 // Output columns / rows of the marching region for strips of 58 and steps of
 // rb rows, rows a multiple of rb.  fast: every output's 3-point level-1 halo
 // inside [max(D,4), n - max(D,4)) in both directions (no buffer, metric or
-// one-sided stencil there).  General: outputs in [9, n - 9), so the U window
-// (6-point halo) stays inside the grid and every derivative in it uses the
-// interior stencil; buffers and metrics are then handled point by point.
+// one-sided stencil there).  General: outputs in [6, n - 6), so the U window
+// (6-point halo) stays inside the grid and every derivative taken in it (U at
+// the level-1 rows and columns, 3 points beyond the outputs) is an interior
+// stencil (3 <= q <= n - 4); buffers and metrics are then handled point by
+// point.
 // False when the region is too small.
+constexpr int GEN_LO = 6;  // first output row / column of the general region
 __host__ __device__ inline bool march_region(bool fast, int D, int imax, int jmax, int rb, MarchMap &m)
 {
-    const int lo = fast ? (D > 4 ? D : 4) + 3 : 9;
+    const int lo = fast ? (D > 4 ? D : 4) + 3 : GEN_LO;
     m.xa = lo;
     m.nstrips = (imax - lo - m.xa) / 58;
     m.ya = lo;
@@ -229,10 +232,10 @@ __host__ __device__ inline bool march_hybrid(int D, int imax, int jmax, int rb, 
     MarchMap f;
     if (!march_region(true, D, imax, jmax, rb, f) || !march_region(false, D, imax, jmax, rb, m)) return false;
     const int lo = f.ya;            // fast zone rows/columns [lo, n - lo)
-    m.ya = lo >= 9 ? 9 + (lo - 9) % rb : lo;  // lo - ya: a multiple of rb
+    m.ya = GEN_LO + (lo - GEN_LO) % rb;  // lo - ya: a multiple of rb
     m.y_a = lo;
     m.y_b = lo + (jmax - 2 * lo) / rb * rb;
-    const int top = jmax - (lo >= 9 ? 9 : lo) - m.y_b;
+    const int top = jmax - GEN_LO - m.y_b;
     m.yb = m.y_b + (top > 0 ? top / rb * rb : 0);
     m.fx0 = lo;
     m.fx1 = imax - lo;

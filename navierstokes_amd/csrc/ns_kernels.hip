@@ -818,7 +818,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
 }
 
 // variants (same arithmetic):
-//   FP64: 0 -> marching 4 rows/step over the general region [9, n-9)^2
+//   FP64: 0 -> marching 4 rows/step over the general region [6, n-6)^2
 //         (buffers and metrics point by point; blocks inside the fast zone
 //         take the fast code), the thin edge band concurrent on s2;
 //         1 -> marching 4 rows/step over the fast zone (the general region

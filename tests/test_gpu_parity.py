@@ -130,7 +130,7 @@ HEAP_ABORT = ("a full GPU-suite run aborted in this test with glibc's 'malloc():
 @pytest.mark.parametrize("name", list(VARIANT_CASES))
 def test_stage_variants(name):
     """The stage's launch variants (fast-zone marching kernel with edge tiles
-    beside it; marching kernel over the whole region 9 in from the edges,
+    beside it; marching kernel over the whole region 6 in from the edges,
     buffers and metrics point by point, with a thin edge band; 8 rows per
     step) compute the same bits in FP64, and match the oracle's steps."""
     case = config.random_case(**VARIANT_CASES[name])
@@ -152,6 +152,37 @@ def test_stage_variants(name):
             ref = U
         else:
             assert np.array_equal(U, ref), f"variant {v}: max diff {np.abs(U - ref).max():.3e}"
+
+
+@pytest.mark.parametrize("name", list(VARIANT_CASES))
+def test_stage_variants_one_context(name):
+    """The stage's launch variants, switched in one context (ns_set_variant
+    between runs from the same initial state): fast-zone marching kernel with
+    edge tiles beside it; marching kernel over the region 6 in from the edges
+    (buffers and metrics point by point, fast blocks inside the fast zone)
+    with a thin edge band; the latter at 8 rows per step.  They compute the
+    same bits in FP64 and match the oracle's steps."""
+    case = config.random_case(**VARIANT_CASES[name])
+    if name.startswith("nonunit"):
+        case.dm, case.dn = 0.8, 1.25   # no fast zone: every variant marches the general region
+    nsteps = 2
+    Uo = Oracle(case).run(case.U0, 0, nsteps)
+    s = _solver(case)
+    ref = None
+    for v in (0, 1, 2):
+        print(f"[variant] {v}", flush=True)
+        s.put_state(case.U0)
+        s.set_variant(v)
+        s.run(0, nsteps)
+        U = s.get_state()
+        errs = [rel_l2(U[..., k], Uo[..., k]) for k in range(case.nk)]
+        print(f"variant {v}: rel L2 vs oracle {max(errs):.3e}", flush=True)
+        assert max(errs) < 1e-13
+        if ref is None:
+            ref = U
+        else:
+            assert np.array_equal(U, ref), f"variant {v}: max diff {np.abs(U - ref).max():.3e}"
+    s.close()
 
 
 def test_source_steps():

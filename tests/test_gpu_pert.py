@@ -104,6 +104,36 @@ def test_pulse_vortex_ns_512():
     s64.close()
 
 
+@pytest.mark.parametrize("nk", [4, 5])
+def test_stage_variants_one_context(nk):
+    """Every stage variant of the perturbation form (interior tiles; marching
+    kernel over the general region; marching kernel over the fast zone),
+    switched in one context from the same initial state, stays within the
+    perturbation bar of FP64, and the variants agree with one another to
+    FP32 rounding."""
+    case = config.random_case(330, 212, nk=nk, D=20, seed=34, amp=1e-3)
+    s64 = _solver(case, FP64)
+    s64.run(0, 10)
+    b = s64.get_state()
+    s64.close()
+    s = _solver(case, FP32P)
+    ref = None
+    for v in (0, 1, 2):
+        print(f"[variant] {v}", flush=True)
+        s.put_state(case.U0)
+        s.set_variant(v)
+        s.run(0, 10)
+        a = s.get_state()
+        errs = [_pert_err(a, b, case.Ub, k) for k in range(nk)]
+        print(f"variant {v}: perturbation relative L2 per field {errs}", flush=True)
+        assert max(errs) < 1e-4
+        if ref is None:
+            ref = a
+        else:
+            assert np.abs(a - ref).max() <= 1e-5 * np.abs(ref - case.Ub).max()
+    s.close()
+
+
 @pytest.mark.skip(reason="same solver sequence (three contexts of one case, each pinned to a stage variant "
                          "before its first step) as the FP64 variant test, which aborted a GPU-suite run with "
                          "host heap corruption of unknown cause")
