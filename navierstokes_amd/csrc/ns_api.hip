@@ -22,9 +22,11 @@
 
 namespace nsamd {
 hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                        hipStream_t s2, StageTiming *tm = nullptr);
+                        hipStream_t s2, StageHooks *tm = nullptr);
 hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                        hipStream_t s2, StageTiming *tm = nullptr);
+                        hipStream_t s2, StageHooks *tm = nullptr);
+hipError_t prepare_step(const StageArgs<double> &A, int nk, bool visc, int variant, bool batch);
+hipError_t prepare_step(const StageArgs<float> &A, int nk, bool visc, int variant, bool batch);
 hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s);
 hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s);
@@ -366,7 +368,7 @@ int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed)
     // stream (fork after the previous work, join before the filters)
     HIPCHK(hipEventRecord(c->fork, c->stream));
     HIPCHK(hipStreamWaitEvent(c->side, c->fork, 0));
-    StageTiming tm;
+    StageHooks tm;
     if (timed) {
         tm.before = c->ev[5];
         tm.after = c->ev[6];
@@ -417,6 +419,18 @@ int enqueue_step_t(ns_ctx *c, int iter, bool timed)
 }
 
 int enqueue_step(ns_ctx *c, int iter, bool timed) { return enqueue_step_t(c, iter, timed); }
+
+// Before a step is captured into a graph: the host-side set-up of its
+// launches (kernel code objects load at first use, marching plans query the
+// device once), so that no such runtime call runs inside the capture.
+template <typename S>
+int prepare_capture_t(ns_ctx *c)
+{
+    const StageArgs<S> A = stage_args<S>(c, 0, static_cast<S *>(c->S[0]), static_cast<S *>(c->S[1]), 0.0, c->pert);
+    HIPCHK(prepare_step(A, c->P.nk, !(c->P.flags & NS_FLAG_INVISCID), c->stage_variant < 0 ? 0 : c->stage_variant,
+                        false));
+    return NS_OK;
+}
 
 // Autotuner: time each variant of the stage launch on the live workload, with
 // the same fork/join onto the side stream as a step, and keep the fastest.
@@ -1082,6 +1096,7 @@ int32_t ns_rk_steps(ns_ctx *c, int32_t iter0, int32_t nsteps)
         return NS_OK;
     }
     if (!c->graph) {
+        if ((rc = c->pert ? prepare_capture_t<float>(c) : prepare_capture_t<double>(c))) return rc;
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_step(c, iter0, false);
@@ -1519,6 +1534,10 @@ int32_t ns_batch_steps(ns_batch *b, int32_t iter0, int32_t nsteps)
     (void)iter0;  // no source term in the batched path: steps do not depend on iter
     if (nsteps < 0) return fail(NS_ERR_PARAM, "nsteps < 0");
     if (!b->graph) {
+        // the batched filter and edge kernels load before the capture (the
+        // stage kernels have run in the autotuner)
+        HIPCHK(b->pert ? prepare_step(StageArgs<float>{}, b->nk, b->visc, 0, true)
+                       : prepare_step(StageArgs<double>{}, b->nk, b->visc, 0, true));
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         rc = batch_enqueue_step(b);

@@ -150,16 +150,18 @@ struct RectSet {
     }
 };
 
-// Optional timing hooks of a stage launch: events recorded on the interior
+// Optional hooks of a stage launch: events recorded on the interior
 // kernel's stream right before and after its launch, and the number of
-// output points that kernel covers.
-struct StageTiming {
+// output points that kernel covers; plan_only: do the host-side set-up
+// (launch plan, LDS attributes, which load the kernels) and launch nothing.
+struct StageHooks {
     hipEvent_t before = nullptr, after = nullptr;
     long long interior_points = 0;
+    bool plan_only = false;
 };
 
 struct MarchMap {
-    int xa, nstrips, ya, yb, H;
+    int xa, xe, nstrips, ya, yb, H;  // output columns [xa, xe): nstrips strips of 58, the last one partial
     int y_a, y_b;
     int fx0, fx1;
 };
@@ -198,8 +200,9 @@ __host__ __device__ inline bool march_block_fast(const MarchMap &m, int i0, int 
     return i0 >= m.fx0 && i0 + 58 <= m.fx1 && y0 >= m.y_a && y1 <= m.y_b;
 }
 
-// Output columns / rows of the marching region for strips of 58 and steps of
-// rb rows, rows a multiple of rb.  fast: every output's 3-point level-1 halo
+// Output columns / rows of the marching region for strips of 58 (the last
+// one partial: its lanes past xe compute from loads clamped to the grid and
+// store nothing) and steps of rb rows, rows a multiple of rb.  fast: every output's 3-point level-1 halo
 // inside [max(D,4), n - max(D,4)) in both directions (no buffer, metric or
 // one-sided stencil there).  General: outputs in [6, n - 6), so the U window
 // (6-point halo) stays inside the grid and every derivative taken in it (U at
@@ -212,7 +215,8 @@ __host__ __device__ inline bool march_region(bool fast, int D, int imax, int jma
 {
     const int lo = fast ? (D > 4 ? D : 4) + 3 : GEN_LO;
     m.xa = lo;
-    m.nstrips = (imax - lo - m.xa) / 58;
+    m.xe = imax - lo;
+    m.nstrips = m.xe > m.xa ? (m.xe - m.xa + 57) / 58 : 0;
     m.ya = lo;
     const int rows = jmax - lo - m.ya;
     m.yb = m.ya + (rows > 0 ? rows / rb * rb : 0);
@@ -220,7 +224,7 @@ __host__ __device__ inline bool march_region(bool fast, int D, int imax, int jma
     m.y_a = m.ya;
     m.y_b = m.yb;
     m.fx0 = m.fx1 = 0;
-    return m.nstrips >= 1 && m.yb - m.ya >= 4 * rb;
+    return m.xe - m.xa >= 58 && m.yb - m.ya >= 4 * rb;
 }
 
 // General region with the fast zone marked (unit grids outside the layers):

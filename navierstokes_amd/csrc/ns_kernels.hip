@@ -486,9 +486,10 @@ static size_t march_smem(int NK, bool visc, int RB)
     return sizeof(R) * ((size_t)(NK + (visc ? 1 : 0)) * WR * MWX + (size_t)NL * LR * MLX);
 }
 
-// one block's march: output columns [i0, i0 + 58), rows [y0, y1)
+// one block's march: output columns [i0, i0 + nout) (nout <= 58), rows [y0, y1)
 template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
-__device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, const int y0, const int y1)
+__device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, const int nout, const int y0,
+                                           const int y1)
 {
     constexpr int NT = RB * 64, WR = MarchRing<RB>::WR, LR = MarchRing<RB>::LR;
     constexpr int NL = L1<NK, VISC>::N;
@@ -505,8 +506,10 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
     const int wy0 = y0 - 6, ly0 = y0 - 3;  // first window / level-1 row of the chunk
     const R gam = R(A.gamma);
 
+    // window columns past the grid (partial last strip) load the last column
+    const int cmax = A.imax - 1 - ox;
     auto load_pt = [&](int r, int c, R *v) {
-        const size_t g = (size_t)r * pitch + (ox + c);
+        const size_t g = (size_t)r * pitch + (ox + (c < cmax ? c : cmax));
 #pragma unroll
         for (int k = 0; k < NK; ++k) v[k] = R(A.U[k][g]);
     };
@@ -556,7 +559,7 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
     // metric factors of this lane's level-1 and output columns (fixed for
     // the whole march)
     constexpr bool MET = PM != PM_FAST;
-    const R mxl = MET ? R(A.metx[lx0 + lane]) : R(1);
+    const R mxl = MET ? R(A.metx[min(lx0 + lane, A.imax - 1)]) : R(1);
     const R mxo = MET ? R(A.metx[min(i0 + lane, A.imax - 1)]) : R(1);
     // wave-uniform row r, column lx0 + lane; my: mety[r]
     auto level1_row = [&](int r, const RowCache<R, NK> *rc, R my) {
@@ -621,7 +624,7 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
         }
         // (c) outputs of row base+wave
         const int gy = base + wave;
-        if (lane < MTX) {
+        if (lane < nout) {
             const int gx = i0 + lane;
             const int ls = (gy - ly0) % LR;
             auto lxa = [=](int l) {
@@ -649,17 +652,18 @@ template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
 __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, MarchMap m)
 {
     const int i0 = m.xa + (int)blockIdx.x * MTX;
+    const int nout = min(MTX, m.xe - i0);
     if constexpr (PM == PM_HYBRID) {
         int y0, y1;
         march_chunk_rows(m, (int)blockIdx.y, y0, y1);
         if (march_block_fast(m, i0, y0, y1))
-            march_body<S, R, NK, VISC, PERT, RB, PM_FAST>(A, i0, y0, y1);
+            march_body<S, R, NK, VISC, PERT, RB, PM_FAST>(A, i0, nout, y0, y1);
         else
-            march_body<S, R, NK, VISC, PERT, RB, PM_MIDU>(A, i0, y0, y1);
+            march_body<S, R, NK, VISC, PERT, RB, PM_MIDU>(A, i0, nout, y0, y1);
     } else {
         // plain layout (y_a = ya, y_b = yb): chunks of H rows
         const int y0 = m.ya + (int)blockIdx.y * m.H;
-        march_body<S, R, NK, VISC, PERT, RB, PM>(A, i0, y0, min(y0 + m.H, m.yb));
+        march_body<S, R, NK, VISC, PERT, RB, PM>(A, i0, nout, y0, min(y0 + m.H, m.yb));
     }
 }
 
@@ -722,7 +726,7 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
 // the two run concurrently; the caller orders s2 after s and joins it back).
 // The RHS-only mode runs the general tiles everywhere.
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT, int NT, int RB, bool MARCH, bool GENERAL = false>
-static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream_t s2, StageTiming *tm)
+static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream_t s2, StageHooks *tm)
 {
     using R = S;
     constexpr int TY = (sizeof(S) == 8 && NK >= 5 && VISC) ? 8 : 16;
@@ -752,7 +756,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
             inner = pm == PM_MIDU ? march_plan<S, NK, VISC, PERT, RB, PM_MIDU>(A, m, nchunks)
                                   : march_plan<S, NK, VISC, PERT, RB, PM_MID>(A, m, nchunks);
         }
-        if (inner) xa = m.xa, xb = m.xa + m.nstrips * MTX, ya = m.ya, yb = m.yb;
+        if (inner) xa = m.xa, xb = m.xe, ya = m.ya, yb = m.yb;
     } else {
         fast_range(RHS_ONLY ? 0 : A.fast_ok, A.D, A.imax, STX, fx0, fx1);
         fast_range(RHS_ONLY ? 0 : A.fast_ok, A.D, A.jmax, TY, fy0, fy1);
@@ -784,33 +788,41 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     };
     if (tm) {
         if constexpr (MARCH && !RHS_ONLY)
-            tm->interior_points = inner ? (long long)m.nstrips * MTX * (m.yb - m.ya) : 0;
+            tm->interior_points = inner ? (long long)(m.xe - m.xa) * (m.yb - m.ya) : 0;
         else
             tm->interior_points = inner ? (long long)(fx1 - fx0) * STX * (fy1 - fy0) * TY : 0;
     }
+    // the interior kernel and its LDS attribute; every runtime call that may
+    // load a kernel happens before the first launch (plan_only stops there)
+    void (*march)(StageArgs<S>, MarchMap) = nullptr;
+    size_t msm = 0;
+    if constexpr (!RHS_ONLY && MARCH) {
+        march = pm == PM_MIDU ? stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MIDU>
+                              : stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MID>;
+        if constexpr (GENERAL) {
+            if (pm == PM_HYBRID) march = stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_HYBRID>;
+        } else {
+            if (pm == PM_FAST) march = stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_FAST>;
+        }
+        msm = march_smem<R>(NK, VISC, RB);
+        if (inner && (e = hipFuncSetAttribute((const void *)march, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)msm)) != hipSuccess)
+            return e;
+    } else if constexpr (!RHS_ONLY) {
+        if (inner && (e = hipFuncSetAttribute((const void *)stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) != hipSuccess)
+            return e;
+    }
+    if (tm && tm->plan_only) return hipSuccess;
     if (!inner) return edges(s);
     if (tm && tm->before && (e = hipEventRecord(tm->before, s)) != hipSuccess) return e;
     if constexpr (!RHS_ONLY && MARCH) {
-        void (*kern)(StageArgs<S>, MarchMap) = pm == PM_MIDU ? stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MIDU>
-                                                              : stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_MID>;
-        if constexpr (GENERAL) {
-            if (pm == PM_HYBRID) kern = stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_HYBRID>;
-        } else {
-            if (pm == PM_FAST) kern = stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_FAST>;
-        }
-        const size_t msm = march_smem<R>(NK, VISC, RB);
-        if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)msm)) !=
-            hipSuccess)
-            return e;
-        hipLaunchKernelGGL(kern, dim3(m.nstrips, nchunks), dim3(RB * 64), msm, s, A, m);
+        hipLaunchKernelGGL(march, dim3(m.nstrips, nchunks), dim3(RB * 64), msm, s, A, m);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (tm && tm->after && (e = hipEventRecord(tm->after, s)) != hipSuccess) return e;
     } else if constexpr (!RHS_ONLY) {
-        auto kern = stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>;
-        if ((e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
-            hipSuccess)
-            return e;
-        hipLaunchKernelGGL(kern, dim3(fx1 - fx0, fy1 - fy0), dim3(NT), smem, s, A, fx0, fy0);
+        hipLaunchKernelGGL((stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>), dim3(fx1 - fx0, fy1 - fy0), dim3(NT),
+                           smem, s, A, fx0, fy0);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (tm && tm->after && (e = hipEventRecord(tm->after, s)) != hipSuccess) return e;
     }
@@ -831,7 +843,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
 // code at unit metric and spacing performs the same operations); in FP32
 // (contraction allowed) the tile and marching kernels may round differently.
 template <typename S, int NK, bool VISC, bool RHS_ONLY, bool PERT>
-static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s, hipStream_t s2, StageTiming *tm)
+static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t s, hipStream_t s2, StageHooks *tm)
 {
     if constexpr (sizeof(S) == 8) {
         switch (variant) {
@@ -850,7 +862,7 @@ static hipError_t launch_stage_v(const StageArgs<S> &A, int variant, hipStream_t
 
 template <typename S, bool PERT>
 static hipError_t launch_stage_p(const StageArgs<S> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                                 hipStream_t s2, StageTiming *tm)
+                                 hipStream_t s2, StageHooks *tm)
 {
     if (nk == 4) {
         if (visc)
@@ -933,7 +945,7 @@ hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, 
 
 #if defined(NS_KERNELS_F64)
 hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                        hipStream_t s2, StageTiming *tm)
+                        hipStream_t s2, StageHooks *tm)
 {
     return launch_stage_p<double, false>(A, nk, visc, rhs_only, variant, s, s2, tm);
 }
@@ -941,7 +953,7 @@ hipError_t launch_stage(const StageArgs<double> &A, int nk, bool visc, bool rhs_
 
 #if defined(NS_KERNELS_F32)
 hipError_t launch_stage(const StageArgs<float> &A, int nk, bool visc, bool rhs_only, int variant, hipStream_t s,
-                        hipStream_t s2, StageTiming *tm)
+                        hipStream_t s2, StageHooks *tm)
 {
     return launch_stage_p<float, true>(A, nk, visc, rhs_only, variant, s, s2, tm);
 }
@@ -1429,6 +1441,51 @@ hipError_t launch_edge_i(const EdgeArgs<float> &E, hipStream_t s) { return launc
 hipError_t launch_edge_batch(const EdgeArgs<float> *args, int ncases, hipStream_t s)
 {
     return launch_edge_batch_t(args, ncases, s);
+}
+#endif
+
+// Host-side set-up of a step before it is captured into a graph.  A kernel's
+// code object loads at its first use, and a marching plan queries the device
+// the first time; such runtime calls must not run inside stream capture.
+// This makes the stage launch's set-up without launching (plan_only), and
+// fetches the attributes of the filter and edge kernels of the single-case
+// (batch = false) or batched step, which loads them.
+template <typename S>
+static hipError_t prepare_step_t(const StageArgs<S> &A, int nk, bool visc, int variant, bool batch)
+{
+    hipError_t e;
+    if (!batch) {
+        StageHooks h;
+        h.plan_only = true;
+        if ((e = launch_stage(A, nk, visc, false, variant, nullptr, nullptr, &h)) != hipSuccess) return e;
+    }
+    const void *kerns[3];
+    if (batch) {
+        kerns[0] = (const void *)filter_x_kernel_batch<S>;
+        kerns[1] = (const void *)filter_y_kernel_batch<S>;
+        kerns[2] = (const void *)edge_i_kernel_batch<S>;
+    } else {
+        kerns[0] = (const void *)filter_x_kernel<S>;
+        kerns[1] = (const void *)filter_y_kernel<S>;
+        kerns[2] = (const void *)edge_i_kernel<S>;
+    }
+    for (const void *k : kerns) {
+        hipFuncAttributes fa;
+        if ((e = hipFuncGetAttributes(&fa, k)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+#if defined(NS_KERNELS_F64)
+hipError_t prepare_step(const StageArgs<double> &A, int nk, bool visc, int variant, bool batch)
+{
+    return prepare_step_t(A, nk, visc, variant, batch);
+}
+#endif
+#if defined(NS_KERNELS_F32)
+hipError_t prepare_step(const StageArgs<float> &A, int nk, bool visc, int variant, bool batch)
+{
+    return prepare_step_t(A, nk, visc, variant, batch);
 }
 #endif
 

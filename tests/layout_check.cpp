@@ -35,29 +35,36 @@ static int check(int imax, int jmax, int D, int rb, int mode, int H, long &layou
     const int Dm = D > 4 ? D : 4;
     std::vector<int> cnt((size_t)imax * jmax, 0);
     const int nch = march_chunks(m);
+    if (m.nstrips != (m.xe - m.xa + 57) / 58 || m.xe > imax) return fail("strips", imax, jmax, D, rb, mode, H);
     for (int s = 0; s < m.nstrips; ++s)
         for (int c = 0; c < nch; ++c) {
             const int i0 = m.xa + 58 * s;
+            const int ie = i0 + 58 < m.xe ? i0 + 58 : m.xe;  // stored outputs [i0, ie); the last strip may be partial
             int y0, y1;
             march_chunk_rows(m, c, y0, y1);
             if (y1 <= y0 || (y1 - y0) % rb) return fail("chunk rows", imax, jmax, D, rb, mode, H);
-            // window rows y0-6 .. y1+5 (level-1 rows y0-3 .. y1+2), columns i0-6 .. i0+63
-            if (y0 - 6 < 0 || y1 + 5 > jmax - 1 || i0 - 6 < 0 || i0 + 63 > imax - 1)
+            // the stored outputs' window: rows y0-6 .. y1+5 (level-1 rows
+            // y0-3 .. y1+2), columns i0-6 .. ie+5 (lanes past ie load clamped
+            // columns and store nothing)
+            if (ie <= i0 || y0 - 6 < 0 || y1 + 5 > jmax - 1 || i0 - 6 < 0 || ie + 5 > imax - 1)
                 return fail("window outside the grid", imax, jmax, D, rb, mode, H);
-            // every derivative of the interior-stencil code is the interior one
-            if (y0 - 3 < 3 || y1 + 2 > jmax - 4 || i0 - 3 < 3 || i0 + 60 > imax - 4)
+            // every derivative the stored outputs depend on is the interior one
+            if (y0 - 3 < 3 || y1 + 2 > jmax - 4 || i0 - 3 < 3 || ie + 2 > imax - 4)
                 return fail("one-sided stencil in a marching block", imax, jmax, D, rb, mode, H);
             const bool fast = mode == 0 || (mode == 2 && march_block_fast(m, i0, y0, y1));
-            if (fast && (i0 - 3 < Dm || i0 + 58 + 3 > imax - Dm || y0 - 3 < Dm || y1 + 3 > jmax - Dm))
+            if (fast && (i0 - 3 < Dm || ie + 3 > imax - Dm || y0 - 3 < Dm || y1 + 3 > jmax - Dm))
                 return fail("fast block outside the fast zone", imax, jmax, D, rb, mode, H);
+            // a hybrid block on the fast code covers whole fast-zone columns
+            if (mode == 2 && march_block_fast(m, i0, y0, y1) && ie != i0 + 58)
+                return fail("partial strip on the fast code", imax, jmax, D, rb, mode, H);
             for (int j = y0; j < y1; ++j)
-                for (int i = i0; i < i0 + 58; ++i) ++cnt[(size_t)j * imax + i];
+                for (int i = i0; i < ie; ++i) ++cnt[(size_t)j * imax + i];
         }
     for (int ty : {8, 16}) {
         std::vector<int> c2 = cnt;
         // bands below and above the marching region in 64 x ty tiles, side
         // strips beside it in 16 x 4ty tiles (the stage launch's edge sets)
-        const int xa = m.xa, xb = m.xa + 58 * m.nstrips;
+        const int xa = m.xa, xb = m.xe;
         const RectSet sets[2] = {RectSet::around(imax, jmax, 64, ty, xa, xb, m.ya, m.yb, 3),
                                  RectSet::around(imax, jmax, 16, 4 * ty, xa, xb, m.ya, m.yb, 12)};
         for (const RectSet &rs : sets)

@@ -120,19 +120,16 @@ VARIANT_CASES = {
 }
 
 
-HEAP_ABORT = ("a full GPU-suite run aborted in this test with glibc's 'malloc(): invalid size' while it "
-              "created its third solver; neither the library's host code (AddressSanitizer, CPU build with a "
-              "stand-in runtime) nor the oracle (AddressSanitizer) shows a cause, so the test stays off the GPU "
-              "until one is found")
-
-
-@pytest.mark.skip(reason=HEAP_ABORT)
 @pytest.mark.parametrize("name", list(VARIANT_CASES))
 def test_stage_variants(name):
     """The stage's launch variants (fast-zone marching kernel with edge tiles
     beside it; marching kernel over the whole region 6 in from the edges,
     buffers and metrics point by point, with a thin edge band; 8 rows per
-    step) compute the same bits in FP64, and match the oracle's steps."""
+    step) compute the same bits in FP64, and match the oracle's steps.  Each
+    context is pinned to its variant before its first step, so its first
+    step graph is captured without an autotuning pass: the kernels and plans
+    are set up before the capture (a kernel first launched inside a capture
+    aborted the HIP runtime)."""
     case = config.random_case(**VARIANT_CASES[name])
     if name.startswith("nonunit"):
         case.dm, case.dn = 0.8, 1.25   # no fast zone: every variant marches the general region

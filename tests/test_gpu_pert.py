@@ -134,9 +134,6 @@ def test_stage_variants_one_context(nk):
     s.close()
 
 
-@pytest.mark.skip(reason="same solver sequence (three contexts of one case, each pinned to a stage variant "
-                         "before its first step) as the FP64 variant test, which aborted a GPU-suite run with "
-                         "host heap corruption of unknown cause")
 @pytest.mark.parametrize("nk", [4, 5])
 def test_stage_variants(nk):
     """Every stage variant of the perturbation form (interior tiles; marching
