@@ -414,24 +414,34 @@ __global__ __launch_bounds__(NT) void stage_kernel_fast(StageArgs<S> A, int bx0,
 }
 
 // Edge tiles around the interior region in one launch: the first
-// sides.count() blocks take SSX x 4 TY tiles over the side strips beside it
-// (a few columns wide: wide tiles there would be mostly empty and 4x as
-// many), the rest STX x TY tiles over the bands below and above it (the
-// whole grid when there is no interior region).  Both shapes use the same
-// LDS box, transposed.  One launch, so that band tiles need not wait for the
+// sides.count() blocks take SX x SY tiles over the side strips beside it (a
+// few columns wide: wide tiles there would be mostly empty), the rest BX x BY
+// tiles over the bands below and above it (the whole grid when there is no
+// interior region).  One launch, so that band tiles need not wait for the
 // last side tile as a second launch on the same stream would.
-constexpr int SSX = 16;
-template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, int NT>
-__global__ __launch_bounds__(NT) void stage_kernel_edges(StageArgs<S> A, RectSet sides, RectSet bands)
+//   large (512 threads): 16 x 4 TY and 64 x TY, the same LDS box transposed,
+//     most of a CU's LDS;
+//   small (256 threads): 16 x 16 and 32 x 8, about a third of it, so one
+//     fits on a CU beside a marching block.
+template <bool SMALL, int TY>
+struct EdgeShape {
+    static constexpr int SX = 16, SY = SMALL ? 16 : 4 * TY;
+    static constexpr int BX = SMALL ? 32 : STX, BY = SMALL ? 8 : TY;
+    static constexpr int NT = SMALL ? 256 : 512;
+};
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, bool SMALL>
+__global__ __launch_bounds__((EdgeShape<SMALL, TY>::NT)) void stage_kernel_edges(StageArgs<S> A, RectSet sides,
+                                                                                RectSet bands)
 {
+    using E = EdgeShape<SMALL, TY>;
     int i0, i1, j0, j1;
     const int t = (int)blockIdx.x;
     if (t < sides.count()) {
         sides.tile(t, i0, i1, j0, j1);
-        stage_tile<S, R, NK, VISC, 4 * TY, RHS_ONLY, PERT, NT, PM_GEN, SSX>(A, i0, i1, j0, j1);
+        stage_tile<S, R, NK, VISC, E::SY, RHS_ONLY, PERT, E::NT, PM_GEN, E::SX>(A, i0, i1, j0, j1);
     } else {
         bands.tile(t - sides.count(), i0, i1, j0, j1);
-        stage_tile<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT, PM_GEN>(A, i0, i1, j0, j1);
+        stage_tile<S, R, NK, VISC, E::BY, RHS_ONLY, PERT, E::NT, PM_GEN, E::BX>(A, i0, i1, j0, j1);
     }
 }
 
@@ -679,7 +689,7 @@ static size_t stage_smem(int NK, bool visc, int TY, int TX = STX)
 // rows a multiple of RB (march_region, march_hybrid); chunk height H from a
 // simple occupancy model (blocks per wave of the GPU x rows per block).
 template <typename S, int NK, bool VISC, bool PERT, int RB, int PM>
-static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
+static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks, bool &one_wave)
 {
     constexpr bool FAST = PM == PM_FAST;
     if ((FAST || PM == PM_HYBRID) && !A.fast_ok) return false;
@@ -717,6 +727,7 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks)
             nchunks = n;
         }
     }
+    one_wave = (long)m.nstrips * nchunks <= slots;
     return true;
 }
 
@@ -735,6 +746,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     int fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
     bool inner = false;
     int pm = PM_GEN;
+    bool one_wave = false;  // every marching block resident at once
     int xa = 0, xb = 0, ya = 0, yb = 0;  // interior region; the edge tiles cover the rest
     if constexpr (MARCH && !RHS_ONLY) {
         // the fast zone (GENERAL: the general region with its fast blocks)
@@ -745,16 +757,16 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         if (A.fast_ok) {
             if constexpr (GENERAL) {
                 pm = PM_HYBRID;
-                inner = march_plan<S, NK, VISC, PERT, RB, PM_HYBRID>(A, m, nchunks);
+                inner = march_plan<S, NK, VISC, PERT, RB, PM_HYBRID>(A, m, nchunks, one_wave);
             } else {
                 pm = PM_FAST;
-                inner = march_plan<S, NK, VISC, PERT, RB, PM_FAST>(A, m, nchunks);
+                inner = march_plan<S, NK, VISC, PERT, RB, PM_FAST>(A, m, nchunks, one_wave);
             }
         }
         if (!inner) {
             pm = gpm;
-            inner = pm == PM_MIDU ? march_plan<S, NK, VISC, PERT, RB, PM_MIDU>(A, m, nchunks)
-                                  : march_plan<S, NK, VISC, PERT, RB, PM_MID>(A, m, nchunks);
+            inner = pm == PM_MIDU ? march_plan<S, NK, VISC, PERT, RB, PM_MIDU>(A, m, nchunks, one_wave)
+                                  : march_plan<S, NK, VISC, PERT, RB, PM_MID>(A, m, nchunks, one_wave);
         }
         if (inner) xa = m.xa, xb = m.xe, ya = m.ya, yb = m.yb;
     } else {
@@ -764,13 +776,33 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         if (inner) xa = fx0 * STX, xb = fx1 * STX, ya = fy0 * TY, yb = fy1 * TY;
     }
     hipError_t e;
-    // edge tiles (stage_kernel_edges): side strips, then the bands
-    const RectSet sides = RectSet::around(A.imax, A.jmax, SSX, 4 * TY, xa, xb, ya, yb, inner ? 12 : 0);
-    const RectSet bands = RectSet::around(A.imax, A.jmax, STX, TY, xa, xb, ya, yb, inner ? 3 : 15);
+    // edge tiles (stage_kernel_edges): side strips, then the bands.  Small
+    // tiles when the marching blocks all run in one wave: LDS freed by a
+    // finished marching block is then not needed again, and a small edge
+    // tile can start there beside the other one.  With more waves that LDS
+    // goes to the next marching blocks, and large tiles wait for whole CUs.
+    using EL = EdgeShape<false, TY>;
+    using ES = EdgeShape<true, TY>;
+    const bool small = inner && one_wave;
+    const RectSet sides = small ? RectSet::around(A.imax, A.jmax, ES::SX, ES::SY, xa, xb, ya, yb, 12)
+                                : RectSet::around(A.imax, A.jmax, EL::SX, EL::SY, xa, xb, ya, yb, inner ? 12 : 0);
+    const RectSet bands = small ? RectSet::around(A.imax, A.jmax, ES::BX, ES::BY, xa, xb, ya, yb, 3)
+                                : RectSet::around(A.imax, A.jmax, EL::BX, EL::BY, xa, xb, ya, yb, inner ? 3 : 15);
     const size_t smem = stage_smem<R>(NK, VISC, TY);
-    const size_t ssm = stage_smem<R>(NK, VISC, 4 * TY, SSX);
-    const size_t esm = ssm > smem ? ssm : smem;
-    auto gen = stage_kernel_edges<S, R, NK, VISC, TY, RHS_ONLY, PERT, NT>;
+    size_t esm, sb;
+    if (small) {
+        esm = stage_smem<R>(NK, VISC, ES::SY, ES::SX);
+        sb = stage_smem<R>(NK, VISC, ES::BY, ES::BX);
+    } else {
+        esm = stage_smem<R>(NK, VISC, EL::SY, EL::SX);
+        sb = stage_smem<R>(NK, VISC, EL::BY, EL::BX);
+    }
+    esm = sb > esm ? sb : esm;
+    void (*gen)(StageArgs<S>, RectSet, RectSet) = stage_kernel_edges<S, R, NK, VISC, TY, RHS_ONLY, PERT, false>;
+    if constexpr (MARCH && !RHS_ONLY) {
+        if (small) gen = stage_kernel_edges<S, R, NK, VISC, TY, RHS_ONLY, PERT, true>;
+    }
+    const int ent = small ? ES::NT : EL::NT;
     if ((e = hipFuncSetAttribute((const void *)gen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)esm)) !=
         hipSuccess)
         return e;
@@ -781,7 +813,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     auto edges = [&](hipStream_t st) -> hipError_t {
         const int n = sides.count() + bands.count();
         if (n > 0) {
-            hipLaunchKernelGGL(gen, dim3(n), dim3(NT), esm, st, A, sides, bands);
+            hipLaunchKernelGGL(gen, dim3(n), dim3(ent), esm, st, A, sides, bands);
             return hipGetLastError();
         }
         return hipSuccess;

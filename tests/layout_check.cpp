@@ -60,13 +60,15 @@ static int check(int imax, int jmax, int D, int rb, int mode, int H, long &layou
             for (int j = y0; j < y1; ++j)
                 for (int i = i0; i < ie; ++i) ++cnt[(size_t)j * imax + i];
         }
-    for (int ty : {8, 16}) {
+    // the stage launch's edge sets: bands below and above the marching
+    // region in 64 x ty tiles, side strips beside it in 16 x 4ty tiles
+    // (large), or 32 x 8 and 16 x 16 (small)
+    const int shapes[3][4] = {{64, 8, 16, 32}, {64, 16, 16, 64}, {32, 8, 16, 16}};
+    for (const auto &sh : shapes) {
         std::vector<int> c2 = cnt;
-        // bands below and above the marching region in 64 x ty tiles, side
-        // strips beside it in 16 x 4ty tiles (the stage launch's edge sets)
         const int xa = m.xa, xb = m.xe;
-        const RectSet sets[2] = {RectSet::around(imax, jmax, 64, ty, xa, xb, m.ya, m.yb, 3),
-                                 RectSet::around(imax, jmax, 16, 4 * ty, xa, xb, m.ya, m.yb, 12)};
+        const RectSet sets[2] = {RectSet::around(imax, jmax, sh[0], sh[1], xa, xb, m.ya, m.yb, 3),
+                                 RectSet::around(imax, jmax, sh[2], sh[3], xa, xb, m.ya, m.yb, 12)};
         for (const RectSet &rs : sets)
             for (int t = 0; t < rs.count(); ++t) {
                 int i0, i1, j0, j1;
