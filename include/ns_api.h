@@ -204,7 +204,7 @@ int32_t ns_vjp_step(ns_ctx *ctx, int32_t iter, const double *U, double *lam);
 /* tuning ---------------------------------------------------------------------- */
 /* Time the stage launch's variants on this context's workload and keep the
    fastest.  FP64 (results bitwise identical across variants): 0 = the
-   interior marching kernel, 4 rows per step, over the whole region 9 points
+   interior marching kernel, 4 rows per step, over the whole region 6 points
    in from the edges (buffers and metrics point by point; blocks inside the
    fast zone, a unit grid outside the buffer/metric layers, take the fast
    code) with a thin edge band beside it on a second stream; 1 = the marching
