@@ -193,6 +193,22 @@ __host__ __device__ inline void march_chunk_rows(const MarchMap &m, int c, int &
     }
 }
 
+// Workgroups of a launch go to the 8 XCDs in turn (block L to XCD L % 8),
+// and each XCD has its own L2.  Block L of T marches tile k = (chunk,
+// strip), strips minor, with the tiles split into 8 contiguous runs, run x
+// for XCD x: the strips an XCD runs side by side share their 6-column
+// halos, and a chunk's 12 prologue rows are the previous chunk's last rows,
+// in the same L2.  A bijection of [0, T) for any T.
+constexpr int NXCD = 8;
+__host__ __device__ inline void march_tile_of(int L, int T, int nstrips, int &strip, int &chunk)
+{
+    const int x = L % NXCD, q = L / NXCD;
+    const int a = T / NXCD, b = T % NXCD;  // run x holds a + (x < b) tiles
+    const int k = x * a + (x < b ? x : b) + q;
+    chunk = k / nstrips;
+    strip = k - chunk * nstrips;
+}
+
 // hybrid layout: the block with outputs [i0, i0 + 58) x [y0, y1) lies in the
 // fast zone and takes the fast per-point code
 __host__ __device__ inline bool march_block_fast(const MarchMap &m, int i0, int y0, int y1)

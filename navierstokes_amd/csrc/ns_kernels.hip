@@ -658,21 +658,30 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
 // others PM_MIDU (the same bits, as for every point mode)
 constexpr int PM_HYBRID = 4;
 
+// this block's (strip, chunk): march_tile_of over the linear block index
+__device__ __forceinline__ void march_tile(int nstrips, int &strip, int &chunk)
+{
+    march_tile_of((int)(blockIdx.x + blockIdx.y * gridDim.x), (int)(gridDim.x * gridDim.y), nstrips, strip,
+                  chunk);
+}
+
 template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
 __global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, MarchMap m)
 {
-    const int i0 = m.xa + (int)blockIdx.x * MTX;
+    int strip, chunk;
+    march_tile(m.nstrips, strip, chunk);
+    const int i0 = m.xa + strip * MTX;
     const int nout = min(MTX, m.xe - i0);
     if constexpr (PM == PM_HYBRID) {
         int y0, y1;
-        march_chunk_rows(m, (int)blockIdx.y, y0, y1);
+        march_chunk_rows(m, chunk, y0, y1);
         if (march_block_fast(m, i0, y0, y1))
             march_body<S, R, NK, VISC, PERT, RB, PM_FAST>(A, i0, nout, y0, y1);
         else
             march_body<S, R, NK, VISC, PERT, RB, PM_MIDU>(A, i0, nout, y0, y1);
     } else {
         // plain layout (y_a = ya, y_b = yb): chunks of H rows
-        const int y0 = m.ya + (int)blockIdx.y * m.H;
+        const int y0 = m.ya + chunk * m.H;
         march_body<S, R, NK, VISC, PERT, RB, PM>(A, i0, nout, y0, min(y0 + m.H, m.yb));
     }
 }

@@ -103,9 +103,33 @@ static int check_whole(int imax, int jmax, int ty)
     return 0;
 }
 
+// the XCD-aware renumbering of marching blocks is a bijection onto the
+// (chunk, strip) tiles, and XCD x's tiles are consecutive
+static int check_tiles(int nstrips, int nchunks)
+{
+    const int T = nstrips * nchunks;
+    std::vector<int> seen(T, 0);
+    std::vector<int> last(NXCD, -1);
+    for (int L = 0; L < T; ++L) {
+        int s, c;
+        march_tile_of(L, T, nstrips, s, c);
+        if (s < 0 || s >= nstrips || c < 0 || c >= nchunks) return fail("tile index", nstrips, nchunks, 0, 0, -2, 0);
+        const int k = c * nstrips + s;
+        ++seen[k];
+        if (last[L % NXCD] >= 0 && k != last[L % NXCD] + 1) return fail("tile run", nstrips, nchunks, 0, 0, -2, 0);
+        last[L % NXCD] = k;
+    }
+    for (int v : seen)
+        if (v != 1) return fail("tile bijection", nstrips, nchunks, 0, 0, -2, 0);
+    return 0;
+}
+
 int main()
 {
     long layouts = 0;
+    for (int ns = 1; ns <= 80; ++ns)
+        for (int nc = 1; nc <= 70; ++nc)
+            if (check_tiles(ns, nc)) return 1;
     const int sizes[] = {40, 77, 128, 150, 201, 256, 301, 330, 512, 700, 1031};
     const int Ds[] = {4, 5, 6, 8, 12, 20, 30, 40, 60};
     for (int imax : sizes)
