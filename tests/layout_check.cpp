@@ -6,8 +6,10 @@
 // every grid point exactly once, every marching block must keep its U window and
 // level-1 rows inside the grid and step a whole number of RB rows, and
 // every block that takes the fast per-point code must have its outputs'
-// 3-point level-1 halo inside the fast zone.  Prints "ok <layouts>" or the
-// first failure, and exits non-zero on failure.
+// 3-point level-1 halo inside the fast zone; and the XCD numbering of the
+// marching blocks (march_tile_of) must be a bijection onto the tiles with
+// each XCD's tiles consecutive.  Prints "ok <layouts>" or the first
+// failure, and exits non-zero on failure.
 #include <cstdio>
 #include <vector>
 
