@@ -217,6 +217,9 @@ int32_t ns_vjp_step(ns_ctx *ctx, int32_t iter, const double *U, double *lam);
    No reference counterpart: the reference has no tunable kernels. */
 int32_t ns_autotune(ns_ctx *ctx, int32_t *variant, double *ms);
 /* Use stage variant 0..2 (as ns_autotune) without timing.
+   Both calls also load every kernel a step of this context launches, as
+   ns_rk_steps does before it captures its step graph (a kernel must not
+   be loaded inside a stream capture).
    No reference counterpart. */
 int32_t ns_set_variant(ns_ctx *ctx, int32_t variant);
 

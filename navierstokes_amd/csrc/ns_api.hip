@@ -514,7 +514,10 @@ int autotune_stage_t(ns_ctx *c)
 
 int autotune_stage(ns_ctx *c)
 {
-    return c->pert ? autotune_stage_t<float>(c) : autotune_stage_t<double>(c);
+    int rc = c->pert ? autotune_stage_t<float>(c) : autotune_stage_t<double>(c);
+    if (rc) return rc;
+    // the chosen variant's set-up, so that a caller may capture the steps
+    return c->pert ? prepare_capture_t<float>(c) : prepare_capture_t<double>(c);
 }
 
 int check_ready(ns_ctx *c)
@@ -1266,7 +1269,8 @@ int32_t ns_set_variant(ns_ctx *c, int32_t variant)
         c->graph = nullptr;
     }
     c->stage_variant = variant;
-    return NS_OK;
+    // set up its kernels now, outside any capture the caller may start
+    return c->pert ? prepare_capture_t<float>(c) : prepare_capture_t<double>(c);
 }
 
 int32_t ns_enable_timing(ns_ctx *c, int32_t on)
