@@ -175,6 +175,10 @@ static int in_ystrip(const nso_params *p, int j1) { return j1 <= p->D || j1 >= p
 
 static double metx_factor_count(const nso_params *p, int i1, int j1, int *cnt)
 {
+    if (p->metric_everywhere) {
+        *cnt = 1;
+        return p->meshx[i1 - 1];
+    }
     const int D = p->D, imax = p->imax, jmax = p->jmax;
     int c = 0;
     if (i1 >= 1 && i1 <= D && j1 >= D + 1 && j1 <= jmax - D) c++;
@@ -189,6 +193,10 @@ static double metx_factor_count(const nso_params *p, int i1, int j1, int *cnt)
 
 static double mety_factor_count(const nso_params *p, int i1, int j1, int *cnt)
 {
+    if (p->metric_everywhere) {
+        *cnt = 1;
+        return p->meshy[j1 - 1];
+    }
     const int D = p->D, imax = p->imax, jmax = p->jmax;
     int c = 0;
     if (i1 >= D + 1 && i1 <= imax - D && j1 >= 1 && j1 <= D) c++;

@@ -29,6 +29,11 @@ typedef struct nso_params {
     double amplitude, wf, r0_acous, m_pt_acous, n_pt_acous;
     /* 1-D profiles (length imax or jmax) and the base flow Ub(nk,imax,jmax) */
     const double *meshx, *meshy, *sigmax, *sigmay, *m, *n, *rhob, *psib, *Ub;
+    /* 0: the reference's metric region tests (diff.f90:276-350); 1: multiply
+       every derivative by meshx(i) / meshy(j) once, at every point (an
+       extension of this build for stretched grids: the reference has no
+       such switch) */
+    int metric_everywhere;
 } nso_params;
 
 /* RHS callback used by nso_rk_euler_cb: dudt = F(U, stage) */

@@ -36,6 +36,7 @@ class NSOParams(C.Structure):
         ("m_pt_acous", C.c_double), ("n_pt_acous", C.c_double),
         ("meshx", _DP), ("meshy", _DP), ("sigmax", _DP), ("sigmay", _DP), ("m", _DP),
         ("n", _DP), ("rhob", _DP), ("psib", _DP), ("Ub", _DP),
+        ("metric_everywhere", C.c_int),
     ]
 
 
@@ -99,6 +100,7 @@ class Oracle:
         p.m_pt_acous, p.n_pt_acous = c.m_pt_acous, c.n_pt_acous
         (p.meshx, p.meshy, p.sigmax, p.sigmay, p.m, p.n, p.rhob, p.psib, p.Ub) = [
             x.ctypes.data_as(_DP) for x in self._keep]
+        p.metric_everywhere = int(bool(getattr(c, "metric_everywhere", False)))
         self.p = p
         self.shape = (c.jmax, c.imax, c.nk)
 

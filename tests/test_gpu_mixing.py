@@ -1,6 +1,7 @@
-"""Mixing layer on the tanh-stretched grid (metrics everywhere): the device
-row average, and the FP32 perturbation form against FP64 through the
-momentum thickness the mixing-layer runs are judged by."""
+"""Mixing layer on the tanh-stretched grid (metrics everywhere): one FP64
+step at the full 2048 x 1024 size against the oracle, the device row
+average, and the FP32 perturbation form against FP64 through the momentum
+thickness the mixing-layer runs are judged by."""
 import numpy as np
 import pytest
 
@@ -14,6 +15,26 @@ def _solver(case, prec):
     from navierstokes_amd.solver import Solver
     print(f"[case] {case.name} precision={prec}", flush=True)
     return Solver(case, precision=prec)
+
+
+def test_full_size_step_vs_oracle():
+    """One FP64 step at 2048 x 1024 (D=60, tanh-stretched y): the
+    general-region march with metrics point by point (point mode 1), the
+    small edge tiles beside it (its blocks fit one wave), against the
+    oracle with the same metric switch."""
+    from oracle.oracle import Oracle
+    case = config.mixing_layer_case(2048, 1024, D=60)
+    Uo = Oracle(case).rk_euler(case.U0, 0)
+    s = _solver(case, 0)
+    s.rk_euler(0)
+    Ug = s.get_state()
+    s.close()
+    for k in range(case.nk):
+        e = rel_l2(Ug[..., k], Uo[..., k])
+        d = rel_l2(Ug[..., k] - case.U0[..., k], Uo[..., k] - case.U0[..., k])
+        print(f"field {k}: rel L2 {e:.3e}, of the step's change {d:.3e}", flush=True)
+        assert e < 1e-13
+        assert d < 1e-11
 
 
 @pytest.mark.parametrize("prec", [0, 1])

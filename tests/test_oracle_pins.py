@@ -282,6 +282,30 @@ def test_rhs_converges_to_navier_stokes():
     assert errs[1] < 2e-4 and errs[0] / errs[1] > 8, errs
 
 
+def test_metric_everywhere_closed_form():
+    """The oracle's metric_everywhere switch (an extension of this build for
+    stretched grids, used by the mixing layer) multiplies every derivative by
+    meshx(i) / meshy(j) exactly once, at every point: bitwise the unit-metric
+    derivative times the profile.  Without it, interior points stay
+    unmultiplied (the reference's region tests, diff.f90:276-350)."""
+    import copy
+    c = config.random_case(40, 36, nk=4, D=6, seed=9)   # random meshx, meshy
+    c.metric_everywhere = True
+    flat = copy.copy(c)
+    flat.meshx = np.ones_like(c.meshx)
+    flat.meshy = np.ones_like(c.meshy)
+    U = c.U0
+    o, of = Oracle(c), Oracle(flat)
+    assert np.array_equal(o.deropm(U), of.deropm(U) * c.meshx[None, :, None])
+    assert np.array_equal(o.deropn(U), of.deropn(U) * c.meshy[:, None, None])
+    ref = copy.copy(c)
+    ref.metric_everywhere = False
+    D = c.D
+    inner = (slice(D + 1, -D - 1), slice(D + 1, -D - 1))
+    assert np.array_equal(Oracle(ref).deropm(U)[inner], of.deropm(U)[inner])
+    assert not np.array_equal(o.deropm(U)[inner], of.deropm(U)[inner])
+
+
 def test_rhs_uniform_state_is_steady():
     c = _plain_case(32, 24)
     U = np.empty((24, 32, 4))
