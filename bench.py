@@ -370,7 +370,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "(multi-GPU sharding is a design note only, not built)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch":
