@@ -155,6 +155,8 @@ struct ns_ctx {
         void *seg = nullptr;      // step states of one segment (K+1)
         size_t segbytes = 0;
         void *stg = nullptr;      // the 6 stage input states of one step
+        void *stgall = nullptr;   // inputs of stages 2-6 of every step of one segment (when they fit)
+        size_t stgallbytes = 0;
         void *lam = nullptr, *lamW = nullptr, *gx = nullptr, *gy = nullptr;
         int iter0 = 0, nsteps = 0, every = 0;
         bool have_fwd = false;
@@ -707,8 +709,8 @@ void release(ns_ctx *c)
     if (c->fork) hipEventDestroy(c->fork);
     if (c->join) hipEventDestroy(c->join);
     if (c->side) hipStreamDestroy(c->side);
-    for (void *p : {(void *)c->adj.arc, (void *)c->adj.jvals, c->adj.ckpt, c->adj.seg, c->adj.stg, c->adj.lam,
-                    c->adj.lamW, c->adj.gx, c->adj.gy})
+    for (void *p : {(void *)c->adj.arc, (void *)c->adj.jvals, c->adj.ckpt, c->adj.seg, c->adj.stg,
+                    c->adj.stgall, c->adj.lam, c->adj.lamW, c->adj.gx, c->adj.gy})
         if (p) hipFree(p);
 }
 
@@ -1771,6 +1773,22 @@ int adj_step_back(ns_ctx *c, int iter, const S *U0, S *lam)
     return NS_OK;
 }
 
+// One step back from kept stage inputs: U0 is the step's state (the input
+// of stage 1), stg5 the inputs of stages 2-6 (adj_backward_t).
+template <typename S>
+int adj_step_back_kept(ns_ctx *c, S *U0, void *stg5, S *lam)
+{
+    // the register W is dead after stage 6 (stage 1 of the next step does not read it)
+    HIPCHK(hipMemsetAsync(c->adj.lamW, 0, state_bytes(c), c->stream));
+    for (int irk = 5; irk >= 0; --irk) {
+        S *in = irk == 0 ? U0 : state_slot<S>(stg5, c, irk - 1);
+        int rc = adj_stage_back<S>(c, irk, in, lam, static_cast<S *>(c->adj.lamW), static_cast<S *>(c->adj.gx),
+                                   static_cast<S *>(c->adj.gy), c->pert, false);
+        if (rc) return rc;
+    }
+    return NS_OK;
+}
+
 template <typename S>
 int adj_forward_t(ns_ctx *c, int iter0, int nsteps, int every, double *J)
 {
@@ -1811,6 +1829,20 @@ int adj_backward_t(ns_ctx *c)
     const int nseg = (N + K - 1) / K;
     int rc = grow(&a.seg, a.segbytes, (size_t)(K + 1) * state_bytes(c));
     if (rc) return rc;
+    // The segment's forward recompute keeps the inputs of stages 2-6 of every
+    // step when they fit in a quarter of the free device memory: a step back
+    // then runs no forward stage again, and the backward costs one forward
+    // step plus the adjoint kernels per step.  Otherwise each step back
+    // recomputes its stage inputs from the step state (adj_step_back).
+    const size_t need = (size_t)K * 5 * state_bytes(c);
+    bool keep = a.stgall && need <= a.stgallbytes;
+    if (!keep) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && need <= fr / 4) {
+            keep = grow(&a.stgall, a.stgallbytes, need) == NS_OK;
+            if (!keep) (void)hipGetLastError();  // the failed allocation is not an error here
+        }
+    }
     S *lam = static_cast<S *>(a.lam);
     S *S0 = static_cast<S *>(c->S[0]);
     HIPCHK(hipMemsetAsync(lam, 0, state_bytes(c), c->stream));
@@ -1819,14 +1851,24 @@ int adj_backward_t(ns_ctx *c)
         HIPCHK(copy_state(c, S0, state_slot<S>(a.ckpt, c, sg)));
         HIPCHK(copy_state(c, state_slot<S>(a.seg, c, 0), S0));
         for (int n = s0; n < s1; ++n) {
-            if ((rc = enqueue_step(c, a.iter0 + n, false))) return rc;
+            if (keep) {
+                for (int irk = 0; irk < 6; ++irk) {
+                    if (irk > 0) HIPCHK(copy_state(c, state_slot<S>(a.stgall, c, (n - s0) * 5 + irk - 1), S0));
+                    if ((rc = enqueue_stage(c, irk, a.iter0 + n, false))) return rc;
+                }
+            } else if ((rc = enqueue_step(c, a.iter0 + n, false))) {
+                return rc;
+            }
             HIPCHK(copy_state(c, state_slot<S>(a.seg, c, n - s0 + 1), S0));
         }
         for (int n = s1 - 1; n >= s0; --n) {
             // J = dt sum_m phi(U_m), m = 1..N: seed dphi/dU at U_{n+1}, then step n back
             HIPCHK(launch_arc_seed(arc_args<S>(c, state_slot<S>(a.seg, c, n - s0 + 1)), lam + 3 * c->plane, c->P.dt,
                                    c->stream));
-            if ((rc = adj_step_back<S>(c, a.iter0 + n, state_slot<S>(a.seg, c, n - s0), lam))) return rc;
+            rc = keep ? adj_step_back_kept<S>(c, state_slot<S>(a.seg, c, n - s0),
+                                              state_slot<S>(a.stgall, c, (n - s0) * 5), lam)
+                      : adj_step_back<S>(c, a.iter0 + n, state_slot<S>(a.seg, c, n - s0), lam);
+            if (rc) return rc;
         }
     }
     HIPCHK(copy_state(c, S0, state_slot<S>(a.ckpt, c, nseg)));
