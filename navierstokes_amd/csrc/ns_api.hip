@@ -353,14 +353,17 @@ double stage_amp(const ns_params &P, int irk, int iter)
 
 // one LDDRK6 stage on the device (rk.f90:52-73, stage irk of step iter):
 // state S0 -> stage kernel -> S1 -> x filter -> S2 -> y filter (+ rows
-// j=1,jmax) -> S0 -> columns i=1,imax
+// j=1,jmax) -> S0 -> columns i=1,imax.  in / out (null: S0) replace S0 as
+// the stage's input and as the y filter's and ccpml columns' output.
 template <typename S>
-int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed)
+int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed, S *in = nullptr, S *out = nullptr)
 {
     const ns_params &P = c->P;
     const bool visc = !(P.flags & NS_FLAG_INVISCID);
     const bool pert = c->pert;
-    S *S0 = static_cast<S *>(c->S[0]), *S1 = static_cast<S *>(c->S[1]), *S2 = static_cast<S *>(c->S[2]);
+    S *S1 = static_cast<S *>(c->S[1]), *S2 = static_cast<S *>(c->S[2]);
+    S *S0 = in ? in : static_cast<S *>(c->S[0]);
+    S *S0out = out ? out : static_cast<S *>(c->S[0]);
     auto tick = [&](int slot) {
         if (timed) hipEventRecord(c->ev[slot], c->stream);
     };
@@ -382,9 +385,9 @@ int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed)
     tick(1);
     HIPCHK(launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, c->stream));
     tick(2);
-    HIPCHK(launch_filter(filter_args<S>(c, false, S2, S0, true, pert), false, c->stream));
+    HIPCHK(launch_filter(filter_args<S>(c, false, S2, S0out, true, pert), false, c->stream));
     tick(3);
-    HIPCHK(launch_edge_i(edge_args<S>(c, S0, pert), c->stream));
+    HIPCHK(launch_edge_i(edge_args<S>(c, S0out, pert), c->stream));
     tick(4);
     if (timed) {
         HIPCHK(hipEventSynchronize(c->ev[4]));
@@ -1852,14 +1855,18 @@ int adj_backward_t(ns_ctx *c)
         HIPCHK(copy_state(c, state_slot<S>(a.seg, c, 0), S0));
         for (int n = s0; n < s1; ++n) {
             if (keep) {
+                // stage irk reads its kept input and writes the next one in place
+                // (the last stage writes the next step state): no copies
                 for (int irk = 0; irk < 6; ++irk) {
-                    if (irk > 0) HIPCHK(copy_state(c, state_slot<S>(a.stgall, c, (n - s0) * 5 + irk - 1), S0));
-                    if ((rc = enqueue_stage(c, irk, a.iter0 + n, false))) return rc;
+                    S *in = irk == 0 ? state_slot<S>(a.seg, c, n - s0) : state_slot<S>(a.stgall, c, (n - s0) * 5 + irk - 1);
+                    S *out = irk == 5 ? state_slot<S>(a.seg, c, n - s0 + 1)
+                                      : state_slot<S>(a.stgall, c, (n - s0) * 5 + irk);
+                    if ((rc = enqueue_stage_t<S>(c, irk, a.iter0 + n, false, in, out))) return rc;
                 }
-            } else if ((rc = enqueue_step(c, a.iter0 + n, false))) {
-                return rc;
+            } else {
+                if ((rc = enqueue_step(c, a.iter0 + n, false))) return rc;
+                HIPCHK(copy_state(c, state_slot<S>(a.seg, c, n - s0 + 1), S0));
             }
-            HIPCHK(copy_state(c, state_slot<S>(a.seg, c, n - s0 + 1), S0));
         }
         for (int n = s1 - 1; n >= s0; --n) {
             // J = dt sum_m phi(U_m), m = 1..N: seed dphi/dU at U_{n+1}, then step n back
