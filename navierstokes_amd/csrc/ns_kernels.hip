@@ -454,7 +454,9 @@ __global__ __launch_bounds__(NT) void stage_kernel_batch(const StageArgs<S> *arg
                                                          int ncases)
 {
     const int4 t = tiles[blockIdx.x];
-    const StageArgs<S> &A = args[(size_t)stage * ncases + t.x];
+    // a copy: through a reference into the table, every store of the tile
+    // could alias the arguments, and their fields would be reloaded
+    const StageArgs<S> A = args[(size_t)stage * ncases + t.x];
     const int i0 = t.y * STX, j0 = t.z * TY;
     stage_tile<S, R, NK, VISC, TY, false, PERT, NT, FAST ? PM_FAST : PM_GEN>(A, i0, min(i0 + STX, A.imax), j0,
                                                                         min(j0 + TY, A.jmax));
