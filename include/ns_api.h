@@ -185,7 +185,10 @@ int32_t ns_get_state_dev(ns_ctx *ctx, double *U);
    keeping a checkpoint every `every` steps, and returns J.  ns_adj_backward
    returns dJ/dU0 (host AoS like ns_get_state; _dev: device planes as
    ns_put_state_dev) by recomputing each segment from its checkpoint; the
-   state is the final state again afterwards. */
+   state is the final state again afterwards.  When 5 * every state-sized
+   buffers fit in a quarter of the free device memory, the backward keeps
+   every stage input of a segment there (held until ns_destroy) instead of
+   recomputing them per step. */
 int32_t ns_adj_set_arc(ns_ctx *ctx, int32_t npts, const int32_t *ij);
 int32_t ns_adj_forward(ns_ctx *ctx, int32_t iter0, int32_t nsteps, int32_t every, double *J);
 int32_t ns_adj_backward(ns_ctx *ctx, double *dJdU0);
