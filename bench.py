@@ -28,7 +28,13 @@ reports both rates and the momentum-thickness growth-rate error of FP32.
 Prints one JSON line: Mpts*steps/s over the timed steps, the roofline of the
 dominant kernel (stage kernel, HBM bound) from HIP events on the library's
 stream, the whole-step roofline against the 704 B/pt/step compulsory
-traffic, and the CPU oracle timed on the host as a reported baseline.
+traffic, and the CPU oracle timed on the host as a reported baseline (an
+OpenMP leg on the workload itself and a one-core leg on the same workload at
+half the grid size, with the host's CPU model and core count).  For the
+ns and pulse workloads in FP64 the same run also times the FP32
+perturbation form on the same workload and steps (`fp32_perturbation`: its
+rate, speed-up over FP64 and relative p' = p - p_b L2 error against the
+FP64 state after the same steps).
 """
 from __future__ import annotations
 
@@ -82,18 +88,51 @@ def profiled_traffic(workload: str, size: int, precision: str, variant: int):
     return None
 
 
-def cpu_baseline(case, steps: int):
-    """The oracle (C restatement of the reference, OpenMP over host cores)
-    timed on a bounded sample of the same workload."""
+def host_cpu():
+    """CPU model and core counts of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": avail, "cpu_count": os.cpu_count()}
+
+
+def cpu_baseline(case, steps: int, serial_case=None, serial_steps: int = 1):
+    """The oracle (C restatement of the reference) timed on the host: an
+    OpenMP build over the host's threads on `steps` steps of the workload
+    itself, and the serial build (one core, like the reference's serial
+    Fortran) on `serial_steps` steps of `serial_case` (the same workload on
+    a smaller grid: a step of the full grid takes about a minute on one
+    core)."""
     from oracle.oracle import Oracle
     o = Oracle(case, parallel=True)
     t0 = time.perf_counter()
     o.run(case.U0, 0, steps)
     dt = time.perf_counter() - t0
-    return {"value": case.npoints * steps / dt / 1e6, "unit": "Mpts·steps/s",
-            "cores": int(o.threads()), "kind": "port",
-            "sample": f"{steps} LDDRK6 step(s) of the same {case.imax}x{case.jmax} workload, "
-                      f"OpenMP build of oracle/ns_oracle.c ({dt:.1f} s)"}
+    out = {"value": case.npoints * steps / dt / 1e6, "unit": "Mpts·steps/s",
+           "cores": int(o.threads()), "kind": "port",
+           "sample": f"{steps} LDDRK6 step(s) of the same {case.imax}x{case.jmax} workload, "
+                     f"OpenMP build of oracle/ns_oracle.c on {o.threads()} threads ({dt:.1f} s)",
+           **host_cpu()}
+    if serial_case is not None:
+        so = Oracle(serial_case, parallel=False)
+        t0 = time.perf_counter()
+        so.run(serial_case.U0, 0, serial_steps)
+        ds = time.perf_counter() - t0
+        out["serial"] = {"value": serial_case.npoints * serial_steps / ds / 1e6, "unit": "Mpts·steps/s",
+                         "cores": 1,
+                         "sample": f"{serial_steps} LDDRK6 step(s) of the same workload on a "
+                                   f"{serial_case.imax}x{serial_case.jmax} grid, serial build of "
+                                   f"oracle/ns_oracle.c ({ds:.1f} s)"}
+    return out
 
 
 def copy_probe(mib: int):
@@ -302,8 +341,8 @@ def bench_mixing(args):
     every = args.sample_every
     case = config.mixing_layer_case(nx, ny, D=60)
     y = config.mixing_layer_rows(case)
-    desc = (f"mixing layer {nx}x{ny}, tanh-stretched y (ratio 10, metrics everywhere), M=0.5, Re_delta=1000, "
-            f"Pr=0.72, D=60, {nsteps} steps, theta every {every}")
+    desc = (f"temporal mixing layer {nx} (periodic x) x {ny} (tanh-stretched y, ratio 10, metrics everywhere), "
+            f"M=0.5, Re_delta=1000, Pr=0.72, 60-row y sponges, {nsteps} steps, theta every {every}")
     print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
     res = {}
     for prec, name in ((0, "fp64"), (1, "fp32pert")):
@@ -313,7 +352,7 @@ def bench_mixing(args):
         times, theta = [], []
         n = 0
         while True:
-            ub = s.row_average(1, case.D + 1, case.imax - case.D)
+            ub = s.row_average(1, *config.mixing_layer_columns(case))
             theta.append(config.momentum_thickness(ub, y, 0.25, -0.25))
             times.append(n * case.dt)
             if n >= nsteps:
@@ -374,6 +413,8 @@ def main():
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serial-baseline", action="store_true", help="skip the one-core leg of the CPU baseline")
+    ap.add_argument("--no-fp32", action="store_true", help="ns/pulse: skip the FP32-perturbation leg")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--variant", type=int, default=-1, help="stage variant 0..2 (default: the autotuned one)")
     ap.add_argument("--precision", choices=["fp64", "fp32pert"], default=None,
@@ -400,37 +441,13 @@ def main():
         return
     args.precision = args.precision or "fp64"
 
-    from navierstokes_amd.solver import Solver
-
     case, desc = make_case(args)
     prec = 1 if args.precision == "fp32pert" else 0
     if prec:
         desc = desc.replace("fp64", "fp32-perturbation")
-    print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
-    s = Solver(case, precision=prec)
-    variant, tune_ms = s.autotune()
-    if args.variant >= 0:
-        s.set_variant(args.variant)
-        variant = args.variant
-    s.run(0, args.warmup)
-    s.synchronize()
-    print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    s.run(args.warmup, args.steps)
-    s.synchronize()
-    t = time.perf_counter() - t0
-    s.check_finite()
-    ms_step = t / args.steps * 1e3
-    value = case.npoints * args.steps / t / 1e6
-
-    # per-kernel device time (HIP events on the library's stream)
-    s.enable_timing(True)
-    s.run(args.warmup + args.steps, args.timing_steps)
-    kt = {name: s.kernel_time(name)[0] for name in ("rhs", "interior", "filter_x", "filter_y", "bc")}
-    npts_int = s.stage_interior_points()
-    s.enable_timing(False)
-    s.close()
-
+    r = run_form(case, desc, prec, args)
+    ms_step, value, kt, npts_int, variant, tune_ms = (r[k] for k in ("ms_step", "value", "kt", "npts_int",
+                                                                     "variant", "tune_ms"))
     nk, elem = case.nk, (4 if prec else 8)
     B_step = alg_bytes_per_point_step(nk, elem)
     B_stage = B_step / 6.0  # per point per stage-kernel launch (average over the 6 stages)
@@ -467,10 +484,59 @@ def main():
                           "frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
     }
+    if not prec and not args.no_fp32:
+        # the metric's second half: the FP32 perturbation form on the same
+        # workload and steps, its speed-up and its p' error against FP64
+        r32 = run_form(case, desc.replace("fp64", "fp32-perturbation"), 1, args)
+        kt32 = r32["kt"]
+        ach32 = alg_bytes_per_point_step(nk, 4) / 6.0 * r32["npts_int"] / (kt32["interior"] * 1e-3) / 1e9
+        out["fp32_perturbation"] = {
+            "ms_per_step": round(r32["ms_step"], 4),
+            "value": round(r32["value"], 3),
+            "speedup_vs_fp64": round(ms_step / r32["ms_step"], 3),
+            "pprime_rel_l2_vs_fp64": float(np.linalg.norm(r32["pprime"] - r["pprime"]) / np.linalg.norm(r["pprime"])),
+            "steps_compared": r["nsteps"],
+            "stage_variant": r32["variant"],
+            "kernel_ms": {k: round(v, 4) for k, v in kt32.items()},
+            "interior_roofline_frac": round(ach32 / HBM_PEAK_GBS, 4),
+        }
     if not args.no_cpu_baseline:
         print(f"[bench] cpu baseline ({args.cpu_steps} steps)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+        serial = make_case(argparse.Namespace(workload=args.workload, size=max(64, case.imax // 2)))[0] \
+            if not args.no_serial_baseline else None
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps, serial)
     print(json.dumps(out), flush=True)
+
+
+def run_form(case, desc, prec, args):
+    """Warm-up, timed steps (wall clock) and HIP-event kernel times of one
+    precision form; returns the rates and the final p - p_b (host)."""
+    from navierstokes_amd.solver import Solver
+    print(f"[bench] {desc}: set-up", file=sys.stderr, flush=True)
+    s = Solver(case, precision=prec)
+    variant, tune_ms = s.autotune()
+    if args.variant >= 0:
+        s.set_variant(args.variant)
+        variant = args.variant
+    s.run(0, args.warmup)
+    s.synchronize()
+    print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    s.run(args.warmup, args.steps)
+    s.synchronize()
+    t = time.perf_counter() - t0
+    s.check_finite()
+    # per-kernel device time (HIP events on the library's stream)
+    s.enable_timing(True)
+    s.run(args.warmup + args.steps, args.timing_steps)
+    kt = {name: s.kernel_time(name)[0] for name in ("rhs", "interior", "filter_x", "filter_y", "bc")}
+    npts_int = s.stage_interior_points()
+    s.enable_timing(False)
+    pprime = s.get_state()[..., 3] - case.Ub[..., 3]
+    s.close()
+    return {"ms_step": t / args.steps * 1e3, "value": case.npoints * args.steps / t / 1e6, "kt": kt,
+            "npts_int": npts_int, "variant": variant, "tune_ms": tune_ms, "pprime": pprime,
+            "nsteps": args.warmup + args.steps + args.timing_steps}
 
 
 if __name__ == "__main__":

@@ -50,6 +50,14 @@ extern "C" {
                                          the species diffusion 1/Pe stays */
 #define NS_FLAG_METRIC_EVERYWHERE 2u  /* apply meshx/meshy at every point (stretched
                                          grids); default = reference region tests */
+#define NS_FLAG_PERIODIC_X 4u         /* periodic in x (no reference counterpart: the
+                                         reference has one-sided x stencils, x buffers
+                                         and inflow/outflow columns, diff.f90:18-36,
+                                         rk.f90:98-104, cc.f90:23-35): wrapped interior
+                                         x stencils, no x buffer or metric layers, no
+                                         inflow/outflow columns, the LDDRK update on
+                                         every column, and the x filter an in-place
+                                         sweep around the ring from i = 1; imax >= 64 */
 
 typedef struct ns_params {
     int32_t imax, jmax; /* grid points (global.f90:12) */
@@ -147,6 +155,33 @@ int32_t ns_ccpml(ns_ctx *ctx, double *U);
 /* rk.f90:171-193 acousticfont(ct,iter): s_acou(imax,jmax) */
 int32_t ns_acousticfont(ns_ctx *ctx, double ct, int32_t iter, double *s_acou);
 
+/* rhs_euler split as the reference splits it, for the drop-in modules
+   derivs, eqeuler and diff (host arrays; T, K1, MI, dT*, psiDiff, d2* and
+   s_acou are imax x jmax, i.e. the reference's (1,imax,jmax) / (imax,jmax);
+   tau, dtau* are (3,imax,jmax); dU* are (nk,imax,jmax)) ----------------- */
+/* derivs.f90:9-46 deriv(U,nk): dUdm, dUdn of U; dTdm, dTdn = K1 * d(T)/dm,
+   K1 * d(T)/dn (after kdT); d2Tdm, d2Tdn their derivatives; for nk = 5
+   psiDiff = U1 alpha_cond (dUdm5 + dUdn5) and d2psidm, d2psidn (may be NULL
+   when nk = 4).  Metric factors as deropm / deropn. */
+int32_t ns_deriv(ns_ctx *ctx, int32_t nk, const double *U, const double *T, const double *K1, double *dUdm,
+                 double *dUdn, double *dTdm, double *dTdn, double *d2Tdm, double *d2Tdn, double *psiDiff,
+                 double *d2psidm, double *d2psidn);
+/* derivs.f90:48-56 kdT(): dTdm, dTdn multiplied by K1 in place */
+int32_t ns_kdt(ns_ctx *ctx, const double *K1, double *dTdm, double *dTdn);
+/* eqeuler.f90:28-47 tau_ij(): tau from MI, dUdm, dUdn (components 2, 3), then
+   dtaudm, dtaudn */
+int32_t ns_tau_ij(ns_ctx *ctx, int32_t nk, const double *MI, const double *dUdm, const double *dUdn, double *tau,
+                  double *dtaudm, double *dtaudn);
+/* eqeuler.f90:53-99 EULERNONLINEAR(DUDT,U,NK) from the fields the calls above
+   leave; d2psi* may be NULL when nk = 4, s_acou NULL for no source */
+int32_t ns_eulernonlinear(ns_ctx *ctx, int32_t nk, double *dUdt, const double *U, const double *dUdm,
+                          const double *dUdn, const double *tau, const double *dtaudm, const double *dtaudn,
+                          const double *d2Tdm, const double *d2Tdn, const double *d2psidm, const double *d2psidn,
+                          const double *s_acou);
+/* diff.f90:315-350 metricsX(dU,nk) / :276-311 metricsY(dU,nk), in place */
+int32_t ns_metrics_x(ns_ctx *ctx, int32_t nk, double *dU);
+int32_t ns_metrics_y(ns_ctx *ctx, int32_t nk, double *dU);
+
 /* batched cases (no reference counterpart: the reference runs one case per
    process) ------------------------------------------------------------------ */
 /* Many independent cases of any grid sizes advanced together.  Each kernel
@@ -187,9 +222,13 @@ int32_t ns_get_state_dev(ns_ctx *ctx, double *U);
    ns_put_state_dev) by recomputing each segment from its checkpoint; the
    state is the final state again afterwards.  When 5 * every state-sized
    buffers fit in a quarter of the free device memory, the backward keeps
-   every stage input of a segment there (held until ns_destroy) instead of
-   recomputing them per step. */
+   every stage input of a segment there instead of recomputing them per
+   step, and frees them when the backward returns. */
 int32_t ns_adj_set_arc(ns_ctx *ctx, int32_t npts, const int32_t *ij);
+/* keep = 0: the backward never keeps stage inputs and recomputes each step's
+   stage inputs from its step state (less memory, more time); keep = 1 (the
+   default): keep them when they fit.  Both give the same bits. */
+int32_t ns_adj_set_keep(ns_ctx *ctx, int32_t keep);
 int32_t ns_adj_forward(ns_ctx *ctx, int32_t iter0, int32_t nsteps, int32_t every, double *J);
 int32_t ns_adj_backward(ns_ctx *ctx, double *dJdU0);
 int32_t ns_adj_backward_dev(ns_ctx *ctx, double *dJdU0);

@@ -23,6 +23,7 @@ NS_PREC_FP32_PERT = 1
 
 NS_FLAG_INVISCID = 1
 NS_FLAG_METRIC_EVERYWHERE = 2
+NS_FLAG_PERIODIC_X = 4
 
 
 class NSParams(C.Structure):
@@ -82,6 +83,12 @@ SIGNATURES = {
     "ns_filter_y": (C.c_int32, [_V, _DP]),
     "ns_ccpml": (C.c_int32, [_V, _DP]),
     "ns_acousticfont": (C.c_int32, [_V, C.c_double, C.c_int32, _DP]),
+    "ns_deriv": (C.c_int32, [_V, C.c_int32] + [_DP] * 12),
+    "ns_kdt": (C.c_int32, [_V, _DP, _DP, _DP]),
+    "ns_tau_ij": (C.c_int32, [_V, C.c_int32] + [_DP] * 6),
+    "ns_eulernonlinear": (C.c_int32, [_V, C.c_int32] + [_DP] * 12),
+    "ns_metrics_x": (C.c_int32, [_V, C.c_int32, _DP]),
+    "ns_metrics_y": (C.c_int32, [_V, C.c_int32, _DP]),
     "ns_autotune": (C.c_int32, [_V, _IP, _DP]),
     "ns_set_variant": (C.c_int32, [_V, C.c_int32]),
     "ns_stage_interior": (C.c_int32, [_V, C.POINTER(C.c_int64)]),
@@ -96,6 +103,7 @@ SIGNATURES = {
     "ns_put_state_dev": (C.c_int32, [_V, _V]),
     "ns_get_state_dev": (C.c_int32, [_V, _V]),
     "ns_adj_set_arc": (C.c_int32, [_V, C.c_int32, _IP]),
+    "ns_adj_set_keep": (C.c_int32, [_V, C.c_int32]),
     "ns_adj_forward": (C.c_int32, [_V, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "ns_adj_backward": (C.c_int32, [_V, _DP]),
     "ns_adj_backward_dev": (C.c_int32, [_V, _V]),

@@ -31,24 +31,37 @@ def _base_planes(solver: Solver, device) -> torch.Tensor:
     return cache
 
 
+def _run_forward(solver: Solver, dp: torch.Tensor, nsteps: int, every: int, iter0: int) -> float:
+    U0 = _base_planes(solver, dp.device).clone()
+    U0[3] += dp
+    torch.cuda.current_stream(dp.device).synchronize()  # the library runs on its own stream
+    solver.put_state_dev(U0)
+    return solver.adj_forward(iter0, nsteps, every)
+
+
 class ObserverObjective(torch.autograd.Function):
     @staticmethod
     def forward(ctx, dp: torch.Tensor, solver: Solver, nsteps: int, every: int, iter0: int = 0):
         c = solver.case
         if tuple(dp.shape) != (c.jmax, c.imax) or dp.dtype != torch.float64 or not dp.is_cuda:
             raise ValueError(f"dp must be a float64 device tensor of shape {(c.jmax, c.imax)}")
-        U0 = _base_planes(solver, dp.device).clone()
-        U0[3] += dp
-        torch.cuda.current_stream(dp.device).synchronize()  # the library runs on its own stream
-        solver.put_state_dev(U0)
-        J = solver.adj_forward(iter0, nsteps, every)
+        J = _run_forward(solver, dp, nsteps, every, iter0)
         ctx.solver = solver
+        ctx.args = (nsteps, every, iter0)
+        # the solver holds the checkpoints of its latest forward only: the
+        # backward re-runs this forward when another one came after it
+        ctx.generation = solver.adj_generation
+        ctx.save_for_backward(dp.detach().clone())
         return torch.tensor(J, dtype=torch.float64, device=dp.device)
 
     @staticmethod
     def backward(ctx, gJ: torch.Tensor):
         solver = ctx.solver
         c = solver.case
+        if solver.adj_generation != ctx.generation:
+            (dp,) = ctx.saved_tensors
+            _run_forward(solver, dp, *ctx.args)
+            ctx.generation = solver.adj_generation
         g = torch.empty((c.nk, c.jmax, c.imax), dtype=torch.float64, device=gJ.device)
         torch.cuda.current_stream(gJ.device).synchronize()
         solver.adj_backward_dev(g)

@@ -71,6 +71,7 @@ class Case:
     n_pt_acous: float = 0.0
     inviscid: bool = False
     metric_everywhere: bool = False
+    periodic_x: bool = False   # no reference counterpart: periodic in x (NS_FLAG_PERIODIC_X)
 
     @property
     def npoints(self) -> int:
@@ -366,14 +367,18 @@ def tanh_stretched_y(jmax: int, Ly: float, ratio: float):
 
 def mixing_layer_case(imax: int = 2048, jmax: int = 1024, D: int = 60, ratio: float = 10.0, delta: float = 1.0,
                       M: float = 0.5, Re_delta: float = 1000.0, Pr: float = 0.72, amp: float = 1e-3,
-                      seed: int = 3, cells_per_delta: float = 20.0, nk: int = 4) -> Case:
-    """Temporal-style compressible mixing layer: uniform x (spacing equal to
-    the finest y spacing, delta / cells_per_delta), tanh-stretched y with
-    spacing ratio `ratio`, metric factors applied everywhere; base flow
-    u = 0.5 M tanh(2y/delta) with the Crocco-Busemann temperature
-    (mixing_layer_fields, init.f90:368-404); seeded perturbation
-    v' = amp N(0,1) exp(-(y/delta)^2); Re based on delta; buffers of D points
-    on all sides."""
+                      seed: int = 3, cells_per_delta: float = 20.0, nk: int = 4, periodic: bool = True) -> Case:
+    """Temporal compressible mixing layer: x periodic and uniform (spacing
+    equal to the finest y spacing, delta / cells_per_delta; period imax
+    spacings), tanh-stretched y with spacing ratio `ratio`, metric factors
+    applied everywhere; base flow u = 0.5 M tanh(2y/delta) with the
+    Crocco-Busemann temperature (mixing_layer_fields, init.f90:368-404);
+    seeded perturbation v' = amp N(0,1) exp(-(y/delta)^2); Re based on
+    delta; D-row sponge buffers at the bottom and the top (the right/top
+    profile of init.f90:158-206, mirrored at the bottom).  periodic=False
+    keeps the reference's x treatment instead: one-sided x stencils, inflow
+    and outflow columns and D-column x buffers (the left one with the
+    reference's sigma = 0, init.f90:168)."""
     rng = np.random.default_rng(seed)
     b = math.acosh(math.sqrt(ratio))
     dn = 2.0 / (jmax - 1)
@@ -383,15 +388,22 @@ def mixing_layer_case(imax: int = 2048, jmax: int = 1024, D: int = 60, ratio: fl
     dm = dy_min
     half = 0.5 * (imax - 1) * dm
     x = np.linspace(-half, half, imax)
-    g = grid_profiles(imax, jmax, -half, half, -1.0, 1.0, D, 0.01, 20.0)
+    g = grid_profiles(imax, jmax, -half, half, -1.0, 1.0, D, 0.01, 20.0, sponge_left_bottom=periodic)
     rhob, psib, Ub = mixing_layer_fields(y, imax, nk, 0.5 * M, -0.5 * M, 1.0, 1.0, delta=delta)
     U0 = Ub.copy()
     U0[..., 2] += amp * rng.standard_normal((jmax, imax)) * np.exp(-(y / delta) ** 2)[:, None]
     dt = 0.5 * dy_min / (1.0 + 0.5 * M)
-    return Case(name=f"mixing-layer-{imax}x{jmax}", imax=imax, jmax=jmax, nk=nk, D=D, dm=dm, dn=dn, dt=dt,
+    name = f"mixing-layer-{imax}x{jmax}" + ("-periodic" if periodic else "")
+    return Case(name=name, imax=imax, jmax=jmax, nk=nk, D=D, dm=dm, dn=dn, dt=dt,
                 sigmaf=0.2, m=x, n=eta, meshx=np.ones(imax), meshy=meshy, sigmax=g["sigmax"], sigmay=g["sigmay"],
                 rhob=rhob, psib=psib, Ub=Ub, U0=U0, Re=Re_delta / delta, Pr=Pr, inviscid=False,
-                metric_everywhere=True, Tref=273.15)
+                metric_everywhere=True, Tref=273.15, periodic_x=periodic)
+
+
+def mixing_layer_columns(case: Case):
+    """1-based column range the x-averaged statistics use: every column
+    when x is periodic, else the columns between the x buffers."""
+    return (1, case.imax) if case.periodic_x else (case.D + 1, case.imax - case.D)
 
 
 def mixing_layer_rows(case: Case, ratio: float = 10.0) -> np.ndarray:

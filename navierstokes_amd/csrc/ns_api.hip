@@ -37,7 +37,7 @@ hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, 
                               int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
 template <typename S>
 hipError_t launch_vorticity(const S *u, const S *v, const double *ub1, int imax, int jmax, int pitch, double inv_dm,
-                            double inv_dn, double *out, hipStream_t s);
+                            double inv_dn, bool xper, double *out, hipStream_t s);
 template <typename S>
 hipError_t launch_station_energy(const S *u, const S *v, const double *ub1, const double *ub2, int pitch, int jmax,
                                  int D, double dn, int nst, const int *cols, double *out, hipStream_t s);
@@ -64,13 +64,13 @@ hipError_t launch_filter_batch(const FilterArgs<float> *args, const int4 *tiles,
 hipError_t launch_edge_batch(const EdgeArgs<double> *args, int ncases, hipStream_t s);
 hipError_t launch_edge_batch(const EdgeArgs<float> *args, int ncases, hipStream_t s);
 hipError_t launch_deriv(const double *const *in, double *const *out, int nk, int imax, int jmax, int pitch, bool ydir,
-                        const double *met, double inv_h, hipStream_t s);
+                        const double *met, double inv_h, bool xper, hipStream_t s);
 hipError_t launch_transport(const double *T, double *MI, double *K1, size_t n, const Transport<double> &tc,
                             hipStream_t s);
 hipError_t launch_rk_steep(double *const *U, const double *const *dudt, double *const *W, int nk, int imax, int jmax,
-                           int pitch, double alpha, double beta, double dt, hipStream_t s);
+                           int pitch, double alpha, double beta, double dt, bool xper, hipStream_t s);
 hipError_t launch_buffer(double *const *U, const double *const *base, int nk, int imax, int jmax, int pitch, int D,
-                         const double *sigx, const double *sigy, hipStream_t s);
+                         const double *sigx, const double *sigy, bool xper, hipStream_t s);
 hipError_t launch_source(double *s_out, int imax, int jmax, double amp, const double *gx, const double *gy,
                          hipStream_t s);
 template <typename S>
@@ -78,6 +78,13 @@ hipError_t launch_probe(const S *const *U, int nk, int pitch, int npts, const in
 template <typename S>
 hipError_t launch_finite(const S *const *U, int nk, int imax, int jmax, int pitch, int *flag, hipStream_t s);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
+hipError_t launch_kdt(const PwArgs &a, const double *k, double *dTdm, double *dTdn, hipStream_t s);
+hipError_t launch_psidiff(const PwArgs &a, const double *rho, const double *pm, const double *pn, double alpha,
+                          double *out, hipStream_t s);
+hipError_t launch_tau(const PwArgs &a, const double *mi, const double *ux, const double *uy, const double *vx,
+                      const double *vy, double *txx, double *txy, double *tyy, hipStream_t s);
+hipError_t launch_euler_pw(const EulerPwArgs &e, hipStream_t s);
+hipError_t launch_metric(const PwArgs &a, double *const *dU, int nk, const double *met, bool ydir, hipStream_t s);
 }  // namespace nsamd
 
 using namespace nsamd;
@@ -97,10 +104,23 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess) return fail(NS_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+// A HIP failure as a status.  hipErrorStreamCaptureUnsupported comes back
+// from a launch path that met a kernel not set up before a stream capture
+// (the kernels' set-up must not run inside one): a call out of order.
+int hip_fail(hipError_t e, const char *what)
+{
+    if (e == hipErrorStreamCaptureUnsupported)
+        return fail(NS_ERR_STATE,
+                    "%s: reached inside a stream capture before its kernels were set up (ns_set_variant, "
+                    "ns_autotune or the first ns_rk_steps of a context set them up)",
+                    what);
+    return fail(NS_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(expr)                                  \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
     } while (0)
 
 // "rhs": the whole stage launch (interior kernel, edge tiles beside it on
@@ -160,6 +180,7 @@ struct ns_ctx {
         void *lam = nullptr, *lamW = nullptr, *gx = nullptr, *gy = nullptr;
         int iter0 = 0, nsteps = 0, every = 0;
         bool have_fwd = false;
+        bool keep = true;  // the backward may keep a segment's stage inputs (ns_adj_set_keep)
     } adj;
 };
 
@@ -299,6 +320,7 @@ StageArgs<S> stage_args(ns_ctx *c, int irk, S *Uin, S *Uout, double amp, bool pe
     A.first = irk == 0;
     A.last = irk == 5;
     A.fast_ok = (P.dm == 1.0 && P.dn == 1.0 && !(P.flags & NS_FLAG_METRIC_EVERYWHERE)) ? 1 : 0;
+    A.xper = (P.flags & NS_FLAG_PERIODIC_X) ? 1 : 0;
     return A;
 }
 
@@ -322,6 +344,7 @@ FilterArgs<S> filter_args(ns_ctx *c, bool xdir, S *in, S *out, bool edge_bc, boo
     F.sigma = P.sigmaf;
     F.nk = P.nk;
     F.edge_bc = edge_bc;
+    F.xper = (P.flags & NS_FLAG_PERIODIC_X) ? 1 : 0;
     return F;
 }
 
@@ -342,6 +365,7 @@ EdgeArgs<S> edge_args(ns_ctx *c, S *U, bool pert)
     E.pitch = c->pitch;
     E.nk = c->P.nk;
     E.gamma = c->P.gamma;
+    E.xper = (c->P.flags & NS_FLAG_PERIODIC_X) ? 1 : 0;
     return E;
 }
 
@@ -387,7 +411,7 @@ int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed, S *in = nullptr, S
     tick(2);
     HIPCHK(launch_filter(filter_args<S>(c, false, S2, S0out, true, pert), false, c->stream));
     tick(3);
-    HIPCHK(launch_edge_i(edge_args<S>(c, S0out, pert), c->stream));
+    if (!(P.flags & NS_FLAG_PERIODIC_X)) HIPCHK(launch_edge_i(edge_args<S>(c, S0out, pert), c->stream));
     tick(4);
     if (timed) {
         HIPCHK(hipEventSynchronize(c->ev[4]));
@@ -530,6 +554,8 @@ int check_ready(ns_ctx *c)
     if (!c) return fail(NS_ERR_PARAM, "null context");
     if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
     if (!c->have_base) return fail(NS_ERR_STATE, "ns_set_base has not been called");
+    // launches and kernel set-up go to the calling thread's current device
+    HIPCHK(hipSetDevice(c->P.device));
     return NS_OK;
 }
 
@@ -651,7 +677,7 @@ int setup_perturbation(ns_ctx *c, const double *Ub)
     // d Ub / dy
     if ((rc = set_ptr_table(c, 0, c->base, nk)) || (rc = set_ptr_table(c, 1, outb, nk))) return rc;
     HIPCHK(launch_deriv((const double *const *)c->dptrs, (double *const *)(c->dptrs + 8), nk, I, J, c->pitch, true,
-                        c->mety, 1.0 / P.dn, c->stream));
+                        c->mety, 1.0 / P.dn, false, c->stream));
     if ((rc = extract_column(c, outb, nk, at(off.dub[0])))) return rc;
     // d Tb / dy
     {
@@ -661,7 +687,7 @@ int setup_perturbation(ns_ctx *c, const double *Ub)
         HIPCHK(hipMemcpyAsync(c->work, tp.data(), tp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
         if ((rc = set_ptr_table(c, 0, c->work, 1))) return rc;
         HIPCHK(launch_deriv((const double *const *)c->dptrs, (double *const *)(c->dptrs + 8), 1, I, J, c->pitch,
-                            true, c->mety, 1.0 / P.dn, c->stream));
+                            true, c->mety, 1.0 / P.dn, false, c->stream));
         if ((rc = extract_column(c, outb, 1, at(off.dtb)))) return rc;
     }
     // Ly(Ub) - Ub
@@ -782,6 +808,8 @@ int32_t ns_create(const ns_params *p, ns_ctx **out)
         return fail(NS_ERR_PARAM, "unknown precision mode %d", P.precision);
     if (!(P.gamma > 1.0) || !(P.Re > 0) || !(P.Pr > 0) || !(P.Pe > 0) || !(P.Tref > 0) || !(P.MI0 > 0) || !(P.K0 > 0))
         return fail(NS_ERR_PARAM, "gamma, Re, Pr, Pe, Tref, MI0, K0 must be positive (gamma > 1)");
+    if ((P.flags & NS_FLAG_PERIODIC_X) && P.imax < 64)
+        return fail(NS_ERR_PARAM, "periodic x needs imax >= 64 (got %d)", P.imax);
     const int W = filter_warmup(P.sigmaf);
     if (W < 0) return fail(NS_ERR_PARAM, "filter recurrence does not decay for sigmaf=%g", P.sigmaf);
 
@@ -907,9 +935,11 @@ int32_t ns_set_profiles(ns_ctx *c, const double *meshx, const double *meshy, con
     double *mety = h.data() + L.mety, *sigy = h.data() + L.sigy, *rb = h.data() + L.rhob;
     double *pb = h.data() + L.psib, *gy = h.data() + L.gy;
     const double r2 = P.r0_acous * P.r0_acous;
+    const bool xper = P.flags & NS_FLAG_PERIODIC_X;
     for (int i = 0; i < imax; ++i) {
         // metricsX (diff.f90:315-350): meshx applies in the x layers only
-        const bool strip = (i + 1 <= D) || (i + 1 >= imax - D + 1);
+        // (periodic x has none)
+        const bool strip = !xper && ((i + 1 <= D) || (i + 1 >= imax - D + 1));
         metx[i] = (everywhere || strip) ? meshx[i] : 1.0;
         sigx[i] = sigmax[i];
         const double dx = m[i] - P.m_pt_acous;
@@ -997,11 +1027,11 @@ int32_t ns_vorticity(ns_ctx *c, double *omega)
     if (c->pert) {
         const float *q = static_cast<const float *>(c->S[0]);
         HIPCHK(launch_vorticity<float>(q + c->plane, q + 2 * c->plane, pert_rows(c).ub[1], P.imax, P.jmax, c->pitch,
-                                       1.0 / P.dm, 1.0 / P.dn, d, c->stream));
+                                       1.0 / P.dm, 1.0 / P.dn, (P.flags & NS_FLAG_PERIODIC_X) != 0, d, c->stream));
     } else {
         const double *U = static_cast<const double *>(c->S[0]);
         HIPCHK(launch_vorticity<double>(U + c->plane, U + 2 * c->plane, nullptr, P.imax, P.jmax, c->pitch,
-                                        1.0 / P.dm, 1.0 / P.dn, d, c->stream));
+                                        1.0 / P.dm, 1.0 / P.dn, (P.flags & NS_FLAG_PERIODIC_X) != 0, d, c->stream));
     }
     HIPCHK(hipMemcpyAsync(omega, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1136,7 +1166,8 @@ static int deriv_common(ns_ctx *c, int nk, double *dU, const double *U, bool ydi
     if (rc) return rc;
     if ((rc = set_ptr_table(c, 0, in, nk)) || (rc = set_ptr_table(c, 1, outb, nk))) return rc;
     HIPCHK(launch_deriv((const double *const *)c->dptrs, (double *const *)(c->dptrs + 8), nk, c->P.imax, c->P.jmax,
-                        c->pitch, ydir, ydir ? c->mety : c->metx, ydir ? 1.0 / c->P.dn : 1.0 / c->P.dm, c->stream));
+                        c->pitch, ydir, ydir ? c->mety : c->metx, ydir ? 1.0 / c->P.dn : 1.0 / c->P.dm,
+                        (c->P.flags & NS_FLAG_PERIODIC_X) != 0, c->stream));
     return download_aos(c, outb, dU, nk);
 }
 
@@ -1184,7 +1215,7 @@ int32_t ns_rk_steep(ns_ctx *c, double alphat, double betat, double *U, const dou
         return rc;
     HIPCHK(launch_rk_steep((double *const *)c->dptrs, (const double *const *)(c->dptrs + 8),
                            (double *const *)(c->dptrs + 16), nk, c->P.imax, c->P.jmax, c->pitch, alphat, betat,
-                           c->P.dt, c->stream));
+                           c->P.dt, (c->P.flags & NS_FLAG_PERIODIC_X) != 0, c->stream));
     if ((rc = download_aos(c, dU, U, nk))) return rc;
     return download_aos(c, dW, w, nk);
 }
@@ -1199,7 +1230,7 @@ int32_t ns_bufferxy(ns_ctx *c, double *U)
     if ((rc = upload_aos(c, dU, U, nk))) return rc;
     if ((rc = set_ptr_table(c, 0, dU, nk)) || (rc = set_ptr_table(c, 1, c->base, nk))) return rc;
     HIPCHK(launch_buffer((double *const *)c->dptrs, (const double *const *)(c->dptrs + 8), nk, c->P.imax, c->P.jmax,
-                         c->pitch, c->P.D, c->sigx, c->sigy, c->stream));
+                         c->pitch, c->P.D, c->sigx, c->sigy, (c->P.flags & NS_FLAG_PERIODIC_X) != 0, c->stream));
     return download_aos(c, dU, U, nk);
 }
 
@@ -1232,7 +1263,8 @@ int32_t ns_ccpml(ns_ctx *c, double *U)
     FilterArgs<double> F = filter_args<double>(c, false, dU, c->work + 8 * c->plane, true, false);
     F.sigma = 0.0;
     HIPCHK(launch_filter(F, false, c->stream));
-    HIPCHK(launch_edge_i(edge_args<double>(c, c->work + 8 * c->plane, false), c->stream));
+    if (!(P.flags & NS_FLAG_PERIODIC_X))
+        HIPCHK(launch_edge_i(edge_args<double>(c, c->work + 8 * c->plane, false), c->stream));
     return download_aos(c, c->work + 8 * c->plane, U, nk);
 }
 
@@ -1248,6 +1280,197 @@ int32_t ns_acousticfont(ns_ctx *c, double ct, int32_t iter, double *s_acou)
     HIPCHK(hipStreamSynchronize(c->stream));
     return NS_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// rhs_euler split the reference's way (derivs.f90, eqeuler.f90,
+// coefficients.f90, diff.f90 metricsX/Y) for the drop-in modules of those
+// names: host arrays in the reference layout, device planes for the work.
+// ---------------------------------------------------------------------------
+namespace {
+
+// nplanes pitched FP64 planes for one call, freed with it
+struct Planes {
+    double *p = nullptr;
+    size_t plane = 0;
+    ~Planes()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(const ns_ctx *c, int n)
+    {
+        plane = c->plane;
+        if (hipMalloc(&p, (size_t)n * plane * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return fail(NS_ERR_NOMEM, "hipMalloc of %d planes failed", n);
+        }
+        return NS_OK;
+    }
+    double *operator[](int k) const { return p + (size_t)k * plane; }
+};
+
+PwArgs pw_args(const ns_ctx *c) { return PwArgs{c->P.imax, c->P.jmax, c->pitch}; }
+
+// derivative of n planes starting at `in` into `out` (x: metricsX factor, y: metricsY)
+int deriv_planes(ns_ctx *c, double *in, double *out, int n, bool ydir)
+{
+    int rc;
+    if ((rc = set_ptr_table(c, 0, in, n)) || (rc = set_ptr_table(c, 1, out, n))) return rc;
+    HIPCHK(launch_deriv((const double *const *)c->dptrs, (double *const *)(c->dptrs + 8), n, c->P.imax, c->P.jmax,
+                        c->pitch, ydir, ydir ? c->mety : c->metx, ydir ? 1.0 / c->P.dn : 1.0 / c->P.dm,
+                        (c->P.flags & NS_FLAG_PERIODIC_X) != 0, c->stream));
+    return NS_OK;
+}
+
+int check_nk(ns_ctx *c, int nk)
+{
+    if (!c) return fail(NS_ERR_PARAM, "null context");
+    if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
+    if (nk < 4 || nk > NKMAX) return fail(NS_ERR_PARAM, "nk must be 4 or 5 (got %d)", nk);
+    return NS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ns_deriv(ns_ctx *c, int32_t nk, const double *U, const double *T, const double *K1, double *dUdm,
+                 double *dUdn, double *dTdm, double *dTdn, double *d2Tdm, double *d2Tdn, double *psiDiff,
+                 double *d2psidm, double *d2psidn)
+{
+    int rc = check_nk(c, nk);
+    if (rc) return rc;
+    if (!U || !T || !K1 || !dUdm || !dUdn || !dTdm || !dTdn || !d2Tdm || !d2Tdn)
+        return fail(NS_ERR_PARAM, "null argument");
+    if (nk >= 5 && (!psiDiff || !d2psidm || !d2psidn)) return fail(NS_ERR_PARAM, "null species argument (nk = 5)");
+    // planes: U, dU/dm, dU/dn (nk each), T, K1, dT/dm, dT/dn, d2T/dm, d2T/dn,
+    // psiDiff, d2psi/dm, d2psi/dn
+    Planes w;
+    if ((rc = w.alloc(c, 3 * nk + 9))) return rc;
+    double *u = w[0], *um = w[nk], *un = w[2 * nk];
+    double *t = w[3 * nk], *k = w[3 * nk + 1], *tm = w[3 * nk + 2], *tn = w[3 * nk + 3];
+    double *t2m = w[3 * nk + 4], *t2n = w[3 * nk + 5], *pd = w[3 * nk + 6], *p2m = w[3 * nk + 7], *p2n = w[3 * nk + 8];
+    if ((rc = upload_aos(c, u, U, nk)) || (rc = upload_aos(c, t, T, 1)) || (rc = upload_aos(c, k, K1, 1))) return rc;
+    // derivs.f90:17-20, :25-26, kdT :28/:48-56, :32-33
+    if ((rc = deriv_planes(c, u, um, nk, false)) || (rc = deriv_planes(c, u, un, nk, true)) ||
+        (rc = deriv_planes(c, t, tm, 1, false)) || (rc = deriv_planes(c, t, tn, 1, true)))
+        return rc;
+    HIPCHK(launch_kdt(pw_args(c), k, tm, tn, c->stream));
+    if ((rc = deriv_planes(c, tm, t2m, 1, false)) || (rc = deriv_planes(c, tn, t2n, 1, true))) return rc;
+    if (nk >= 5) {  // derivs.f90:36-43
+        HIPCHK(launch_psidiff(pw_args(c), u, um + 4 * c->plane, un + 4 * c->plane, c->P.alpha_cond, pd, c->stream));
+        if ((rc = deriv_planes(c, pd, p2m, 1, false)) || (rc = deriv_planes(c, pd, p2n, 1, true))) return rc;
+    }
+    if ((rc = download_aos(c, um, dUdm, nk)) || (rc = download_aos(c, un, dUdn, nk)) ||
+        (rc = download_aos(c, tm, dTdm, 1)) || (rc = download_aos(c, tn, dTdn, 1)) ||
+        (rc = download_aos(c, t2m, d2Tdm, 1)) || (rc = download_aos(c, t2n, d2Tdn, 1)))
+        return rc;
+    if (nk >= 5 && ((rc = download_aos(c, pd, psiDiff, 1)) || (rc = download_aos(c, p2m, d2psidm, 1)) ||
+                    (rc = download_aos(c, p2n, d2psidn, 1))))
+        return rc;
+    return NS_OK;
+}
+
+int32_t ns_kdt(ns_ctx *c, const double *K1, double *dTdm, double *dTdn)
+{
+    if (!c || !K1 || !dTdm || !dTdn) return fail(NS_ERR_PARAM, "null argument");
+    Planes w;
+    int rc;
+    if ((rc = w.alloc(c, 3))) return rc;
+    if ((rc = upload_aos(c, w[0], K1, 1)) || (rc = upload_aos(c, w[1], dTdm, 1)) || (rc = upload_aos(c, w[2], dTdn, 1)))
+        return rc;
+    HIPCHK(launch_kdt(pw_args(c), w[0], w[1], w[2], c->stream));
+    if ((rc = download_aos(c, w[1], dTdm, 1))) return rc;
+    return download_aos(c, w[2], dTdn, 1);
+}
+
+int32_t ns_tau_ij(ns_ctx *c, int32_t nk, const double *MI, const double *dUdm, const double *dUdn, double *tau,
+                  double *dtaudm, double *dtaudn)
+{
+    int rc = check_nk(c, nk);
+    if (rc) return rc;
+    if (!MI || !dUdm || !dUdn || !tau || !dtaudm || !dtaudn) return fail(NS_ERR_PARAM, "null argument");
+    Planes w;  // MI, dU/dm (nk), dU/dn (nk), tau (3), dtau/dm (3), dtau/dn (3)
+    if ((rc = w.alloc(c, 2 * nk + 10))) return rc;
+    double *mi = w[0], *um = w[1], *un = w[1 + nk], *tu = w[1 + 2 * nk], *tm = w[4 + 2 * nk], *tn = w[7 + 2 * nk];
+    if ((rc = upload_aos(c, mi, MI, 1)) || (rc = upload_aos(c, um, dUdm, nk)) || (rc = upload_aos(c, un, dUdn, nk)))
+        return rc;
+    const size_t P = c->plane;
+    HIPCHK(launch_tau(pw_args(c), mi, um + P, un + P, um + 2 * P, un + 2 * P, tu, tu + P, tu + 2 * P, c->stream));
+    // eqeuler.f90:43-45
+    if ((rc = deriv_planes(c, tu, tm, 3, false)) || (rc = deriv_planes(c, tu, tn, 3, true))) return rc;
+    if ((rc = download_aos(c, tu, tau, 3)) || (rc = download_aos(c, tm, dtaudm, 3))) return rc;
+    return download_aos(c, tn, dtaudn, 3);
+}
+
+int32_t ns_eulernonlinear(ns_ctx *c, int32_t nk, double *dUdt, const double *U, const double *dUdm,
+                          const double *dUdn, const double *tau, const double *dtaudm, const double *dtaudn,
+                          const double *d2Tdm, const double *d2Tdn, const double *d2psidm, const double *d2psidn,
+                          const double *s_acou)
+{
+    int rc = check_nk(c, nk);
+    if (rc) return rc;
+    if (!dUdt || !U || !dUdm || !dUdn || !tau || !dtaudm || !dtaudn || !d2Tdm || !d2Tdn)
+        return fail(NS_ERR_PARAM, "null argument");
+    if (nk >= 5 && (!d2psidm || !d2psidn)) return fail(NS_ERR_PARAM, "null species argument (nk = 5)");
+    // U, dU/dm, dU/dn, out (nk each), tau, dtau/dm, dtau/dn (3 each), d2T/dm,
+    // d2T/dn, d2psi/dm, d2psi/dn, s_acou
+    Planes w;
+    if ((rc = w.alloc(c, 4 * nk + 14))) return rc;
+    const size_t P = c->plane;
+    double *u = w[0], *um = w[nk], *un = w[2 * nk], *out = w[3 * nk], *tu = w[4 * nk], *tm = w[4 * nk + 3];
+    double *tn = w[4 * nk + 6], *t2m = w[4 * nk + 9], *t2n = w[4 * nk + 10], *p2m = w[4 * nk + 11];
+    double *p2n = w[4 * nk + 12], *src = w[4 * nk + 13];
+    if ((rc = upload_aos(c, u, U, nk)) || (rc = upload_aos(c, um, dUdm, nk)) || (rc = upload_aos(c, un, dUdn, nk)) ||
+        (rc = upload_aos(c, tu, tau, 3)) || (rc = upload_aos(c, tm, dtaudm, 3)) || (rc = upload_aos(c, tn, dtaudn, 3)) ||
+        (rc = upload_aos(c, t2m, d2Tdm, 1)) || (rc = upload_aos(c, t2n, d2Tdn, 1)))
+        return rc;
+    if (nk >= 5 && ((rc = upload_aos(c, p2m, d2psidm, 1)) || (rc = upload_aos(c, p2n, d2psidn, 1)))) return rc;
+    if (s_acou)  // s_acou(imax, jmax): unpitched
+        HIPCHK(hipMemcpyAsync(src, s_acou, sizeof(double) * c->P.imax * c->P.jmax, hipMemcpyHostToDevice, c->stream));
+    EulerPwArgs e{};
+    e.a = pw_args(c);
+    e.nk = nk;
+    for (int k = 0; k < nk; ++k) {
+        e.U[k] = u + k * P;
+        e.Um[k] = um + k * P;
+        e.Un[k] = un + k * P;
+        e.out[k] = out + k * P;
+    }
+    for (int k = 0; k < 3; ++k) {
+        e.tau[k] = tu + k * P;
+        e.taum[k] = tm + k * P;
+        e.taun[k] = tn + k * P;
+    }
+    e.d2Tm = t2m;
+    e.d2Tn = t2n;
+    e.d2pm = nk >= 5 ? p2m : nullptr;
+    e.d2pn = nk >= 5 ? p2n : nullptr;
+    e.src = s_acou ? src : nullptr;
+    e.gamma = c->P.gamma;
+    e.Re = c->P.Re;
+    e.Pr = c->P.Pr;
+    e.Pe = c->P.Pe;
+    HIPCHK(launch_euler_pw(e, c->stream));
+    return download_aos(c, out, dUdt, nk);
+}
+
+static int metrics_common(ns_ctx *c, int32_t nk, double *dU, bool ydir)
+{
+    if (!c || !dU) return fail(NS_ERR_PARAM, "null argument");
+    if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
+    if (nk < 1 || nk > 8) return fail(NS_ERR_PARAM, "nk must lie in 1..8");
+    Planes w;
+    int rc;
+    if ((rc = w.alloc(c, nk)) || (rc = upload_aos(c, w[0], dU, nk)) || (rc = set_ptr_table(c, 0, w[0], nk))) return rc;
+    HIPCHK(launch_metric(pw_args(c), (double *const *)c->dptrs, nk, ydir ? c->mety : c->metx, ydir, c->stream));
+    return download_aos(c, w[0], dU, nk);
+}
+
+int32_t ns_metrics_x(ns_ctx *c, int32_t nk, double *dU) { return metrics_common(c, nk, dU, false); }
+int32_t ns_metrics_y(ns_ctx *c, int32_t nk, double *dU) { return metrics_common(c, nk, dU, true); }
 
 int32_t ns_autotune(ns_ctx *c, int32_t *variant, double *ms)
 {
@@ -1635,6 +1858,7 @@ int adj_check(ns_ctx *c)
     if (!c->have_state) return fail(NS_ERR_STATE, "no state has been set");
     if (c->P.nk != 4 || !(c->P.flags & NS_FLAG_INVISCID))
         return fail(NS_ERR_PARAM, "the adjoint covers the 2-D Euler equations (nk = 4 with NS_FLAG_INVISCID)");
+    if (c->P.flags & NS_FLAG_PERIODIC_X) return fail(NS_ERR_PARAM, "the adjoint does not cover periodic x");
     return NS_OK;
 }
 
@@ -1833,19 +2057,39 @@ int adj_backward_t(ns_ctx *c)
     int rc = grow(&a.seg, a.segbytes, (size_t)(K + 1) * state_bytes(c));
     if (rc) return rc;
     // The segment's forward recompute keeps the inputs of stages 2-6 of every
-    // step when they fit in a quarter of the free device memory: a step back
-    // then runs no forward stage again, and the backward costs one forward
-    // step plus the adjoint kernels per step.  Otherwise each step back
-    // recomputes its stage inputs from the step state (adj_step_back).
+    // step when they fit in a quarter of the free device memory (and the
+    // caller has not switched it off): a step back then runs no forward
+    // stage again, and the backward costs one forward step plus the adjoint
+    // kernels per step.  Otherwise each step back recomputes its stage
+    // inputs from the step state (adj_step_back).  The kept inputs are
+    // released when the backward ends.
     const size_t need = (size_t)K * 5 * state_bytes(c);
-    bool keep = a.stgall && need <= a.stgallbytes;
-    if (!keep) {
+    bool keep = false;
+    if (a.keep) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess && need <= fr / 4) {
-            keep = grow(&a.stgall, a.stgallbytes, need) == NS_OK;
-            if (!keep) (void)hipGetLastError();  // the failed allocation is not an error here
+            void *p = nullptr;
+            if (hipMalloc(&p, need) == hipSuccess) {
+                a.stgall = p;
+                a.stgallbytes = need;
+                keep = true;
+            } else {
+                (void)hipGetLastError();  // the optional buffer did not fit: not an error
+            }
         }
     }
+    struct Release {  // also on an error return: wait for the stream, then free
+        ns_ctx::Adj &a;
+        hipStream_t s;
+        ~Release()
+        {
+            if (!a.stgall) return;
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(a.stgall);
+            a.stgall = nullptr;
+            a.stgallbytes = 0;
+        }
+    } release_kept{a, c->stream};
     S *lam = static_cast<S *>(a.lam);
     S *S0 = static_cast<S *>(c->S[0]);
     HIPCHK(hipMemsetAsync(lam, 0, state_bytes(c), c->stream));
@@ -1879,6 +2123,7 @@ int adj_backward_t(ns_ctx *c)
         }
     }
     HIPCHK(copy_state(c, S0, state_slot<S>(a.ckpt, c, nseg)));
+    // the kept inputs are freed on return: the stream must be done with them
     HIPCHK(hipStreamSynchronize(c->stream));
     return NS_OK;
 }
@@ -2009,6 +2254,13 @@ int32_t ns_adj_set_arc(ns_ctx *c, int32_t npts, const int32_t *ij)
     return NS_OK;
 }
 
+int32_t ns_adj_set_keep(ns_ctx *c, int32_t keep)
+{
+    if (!c) return fail(NS_ERR_PARAM, "null context");
+    c->adj.keep = keep != 0;
+    return NS_OK;
+}
+
 int32_t ns_adj_forward(ns_ctx *c, int32_t iter0, int32_t nsteps, int32_t every, double *J)
 {
     int rc = adj_check(c);
@@ -2076,6 +2328,7 @@ int32_t ns_vjp_rhs(ns_ctx *c, const double *U, const double *lam_f, double *lam_
     if (!U || !lam_f || !lam_U) return fail(NS_ERR_PARAM, "null argument");
     if (c->P.nk != 4 || !(c->P.flags & NS_FLAG_INVISCID))
         return fail(NS_ERR_PARAM, "the adjoint covers the 2-D Euler equations (nk = 4 with NS_FLAG_INVISCID)");
+    if (c->P.flags & NS_FLAG_PERIODIC_X) return fail(NS_ERR_PARAM, "the adjoint does not cover periodic x");
     // FP64 operator mirror: U, lam and the flux adjoints in the FP64 work planes
     double *dU = c->work, *dl = c->work + 4 * c->plane, *gx = c->work + 8 * c->plane, *gy = c->work + 12 * c->plane;
     if ((rc = upload_aos(c, dU, U, 4)) || (rc = upload_aos(c, dl, lam_f, 4))) return rc;
@@ -2086,6 +2339,7 @@ int32_t ns_vjp_rhs(ns_ctx *c, const double *U, const double *lam_f, double *lam_
 static int vjp_linear(ns_ctx *c, double *lam, bool edge, bool fy, bool fx)
 {
     if (!c || !lam) return fail(NS_ERR_PARAM, "null argument");
+    if (c->P.flags & NS_FLAG_PERIODIC_X) return fail(NS_ERR_PARAM, "the adjoint does not cover periodic x");
     if (!c->have_profiles) return fail(NS_ERR_STATE, "ns_set_profiles has not been called");
     double *d = c->work, *tmp = c->work + 8 * c->plane;
     int rc;

@@ -45,7 +45,12 @@ struct StageArgs {
     Transport<double> tc;
     int first, last;  // stage 1 (alpha=0: W not read), stage 6 (W not written)
     int fast_ok;      // unit grid, region metrics: interior tiles may take the fast path
+    int xper;         // periodic in x: wrapped x stencils, no x buffer layers, every column updated
 };
+
+// Column x of a grid of n columns, wrapped into [0, n) (periodic x; x may
+// lie up to one period outside).
+__host__ __device__ inline int wrap_col(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
 
 // ccpml constants of the rows j=1 / j=jmax (cc.f90:38-59): components 0-2
 // extrapolate (plus a base residual in the perturbation form), 3-4 are fixed.
@@ -68,7 +73,27 @@ struct FilterArgs {
     double sigma;
     int nk;
     int edge_bc;  // y filter only: write the j=1 / j=jmax rows of ccpml
+    int xper;     // periodic x: the x filter sweeps around the ring; the y
+                  // filter's ccpml rows cover every column
     RowEdge re;
+};
+
+// Pointwise pieces of rhs_euler on pitched planes (operator-level API).
+struct PwArgs {
+    int imax, jmax, pitch;
+};
+
+// EULERNONLINEAR (eqeuler.f90:61-97): U, dU/dm, dU/dn nk planes each; tau,
+// dtau/dm, dtau/dn 3 planes each; d2p* (nk = 5 only) and src (s_acou,
+// imax x jmax unpitched) may be null
+struct EulerPwArgs {
+    PwArgs a;
+    int nk;
+    const double *U[NKMAX], *Um[NKMAX], *Un[NKMAX];
+    const double *tau[3], *taum[3], *taun[3];
+    const double *d2Tm, *d2Tn, *d2pm, *d2pn, *src;
+    double gamma, Re, Pr, Pe;
+    double *out[NKMAX];
 };
 
 template <typename S>
@@ -79,6 +104,7 @@ struct EdgeArgs {
     RowEdge re;
     int imax, jmax, pitch, nk;
     double gamma;
+    int xper;  // periodic x: no inflow / outflow columns (the kernel does nothing)
 };
 
 // Filter chunk bounds: chunks of length L; a chunk that would end within 16
@@ -226,12 +252,15 @@ __host__ __device__ inline bool march_block_fast(const MarchMap &m, int i0, int 
 // stencil (3 <= q <= n - 4); buffers and metrics are then handled point by
 // point.
 // False when the region is too small.
+// Periodic x (xper): every column is an output column (the window columns
+// wrap around the period) and there are no x layers.
 constexpr int GEN_LO = 6;  // first output row / column of the general region
-__host__ __device__ inline bool march_region(bool fast, int D, int imax, int jmax, int rb, MarchMap &m)
+__host__ __device__ inline bool march_region(bool fast, int D, int imax, int jmax, int rb, MarchMap &m,
+                                             bool xper = false)
 {
     const int lo = fast ? (D > 4 ? D : 4) + 3 : GEN_LO;
-    m.xa = lo;
-    m.xe = imax - lo;
+    m.xa = xper ? 0 : lo;
+    m.xe = xper ? imax : imax - lo;
     m.nstrips = m.xe > m.xa ? (m.xe - m.xa + 57) / 58 : 0;
     m.ya = lo;
     const int rows = jmax - lo - m.ya;
@@ -247,18 +276,19 @@ __host__ __device__ inline bool march_region(bool fast, int D, int imax, int jma
 // columns as the general region; rows split at the fast zone's bounds, each
 // part a multiple of rb.  False when there is no fast zone of at least 4 rb
 // rows; then the plain general region applies.
-__host__ __device__ inline bool march_hybrid(int D, int imax, int jmax, int rb, MarchMap &m)
+__host__ __device__ inline bool march_hybrid(int D, int imax, int jmax, int rb, MarchMap &m, bool xper = false)
 {
     MarchMap f;
-    if (!march_region(true, D, imax, jmax, rb, f) || !march_region(false, D, imax, jmax, rb, m)) return false;
+    if (!march_region(true, D, imax, jmax, rb, f, xper) || !march_region(false, D, imax, jmax, rb, m, xper))
+        return false;
     const int lo = f.ya;            // fast zone rows/columns [lo, n - lo)
     m.ya = GEN_LO + (lo - GEN_LO) % rb;  // lo - ya: a multiple of rb
     m.y_a = lo;
     m.y_b = lo + (jmax - 2 * lo) / rb * rb;
     const int top = jmax - GEN_LO - m.y_b;
     m.yb = m.y_b + (top > 0 ? top / rb * rb : 0);
-    m.fx0 = lo;
-    m.fx1 = imax - lo;
+    m.fx0 = xper ? 0 : lo;
+    m.fx1 = xper ? imax : imax - lo;
     return m.y_b - m.y_a >= 4 * rb;
 }
 

@@ -15,9 +15,66 @@
 // (the FP32 perturbation form, contraction allowed).
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "ns_kernels.h"
 
 namespace nsamd {
+
+// ---------------------------------------------------------------------------
+// Kernel set-up before launch.  The first use of a kernel loads its code
+// object, and a launch path may set its dynamic-LDS limit and query its
+// occupancy and the CU count.  Those runtime calls must not run inside a
+// stream capture (a kernel first loaded inside a capture aborted the HIP
+// runtime), so each (device, kernel) is set up once, outside any capture,
+// and remembered here.  A launch path that reaches a kernel not yet set up
+// while its stream is capturing returns hipErrorStreamCaptureUnsupported
+// (the library reports NS_ERR_STATE) instead of making those calls.
+// ---------------------------------------------------------------------------
+struct KernelSetup {
+    int ncu = 0;     // compute units of the device
+    int perblk = 0;  // resident blocks per CU at the set-up's block size and LDS
+};
+
+static hipError_t kernel_ready(const void *kern, size_t dyn_smem, int threads, hipStream_t s,
+                               KernelSetup *out = nullptr)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, KernelSetup> done;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = done.find({dev, kern});
+        if (it != done.end()) {
+            if (out) *out = it->second;
+            return hipSuccess;
+        }
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return hipErrorStreamCaptureUnsupported;
+    if (dyn_smem > 0 &&
+        (e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_smem)) != hipSuccess)
+        return e;
+    hipFuncAttributes fa;
+    if ((e = hipFuncGetAttributes(&fa, kern)) != hipSuccess) return e;
+    KernelSetup k;
+    if (hipDeviceGetAttribute(&k.ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || k.ncu <= 0)
+        k.ncu = 256;
+    if (threads > 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k.perblk, kern, threads, dyn_smem) !=
+                            hipSuccess ||
+                        k.perblk < 1))
+        k.perblk = 1;
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> g(mu);
+    done[{dev, kern}] = k;
+    if (out) *out = k;
+    return hipSuccess;
+}
 
 // ---------------------------------------------------------------------------
 // Stage kernel: LDS-tiled RHS with nested stencils, RK update and buffer.
@@ -73,12 +130,14 @@ __device__ __forceinline__ R drp_interior_unit(const F &f, int q)
 // ---------------------------------------------------------------------------
 constexpr int PM_GEN = 0, PM_MID = 1, PM_MIDU = 2, PM_FAST = 3;
 
+// per: the line is periodic (x with xper): the interior stencil everywhere,
+// the accessor wraps
 template <typename R, int PM, typename F>
-__device__ __forceinline__ R dv(const F &f, int q, int n, R inv_h)
+__device__ __forceinline__ R dv(const F &f, int q, int n, R inv_h, bool per = false)
 {
     if constexpr (PM >= PM_MIDU) return drp_interior_unit<R>(f, q);
     else if constexpr (PM == PM_MID) return drp_interior<R>(f, q, inv_h);
-    else return drp_deriv<R>(f, q, n, inv_h);
+    else return per ? drp_interior<R>(f, q, inv_h) : drp_deriv<R>(f, q, n, inv_h);
 }
 
 template <typename R, int PM>
@@ -129,7 +188,7 @@ __device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transp
     const int imax = A.imax, jmax = A.jmax;
     const R inv_dm = R(A.inv_dm), inv_dn = R(A.inv_dn);
     if constexpr (NK >= 5) {
-        const R psx = met<R, PM>(dv<R, PM>(ux(4), gx, imax, inv_dm), mx);
+        const R psx = met<R, PM>(dv<R, PM>(ux(4), gx, imax, inv_dm, A.xper), mx);
         R psy = met<R, PM>(dv<R, PM>(uy(4), gy, jmax, inv_dn), my);
         R rho = ux(0)(gx);
         if constexpr (PERT) {
@@ -139,11 +198,11 @@ __device__ __forceinline__ void level1_point(const StageArgs<S> &A, const Transp
         out[L1<NK, VISC>::PSI] = rho * R(A.alpha_cond) * (psx + psy);  // derivs.f90:40
     }
     if constexpr (VISC) {
-        const R uxv = met<R, PM>(dv<R, PM>(ux(1), gx, imax, inv_dm), mx);
-        const R vxv = met<R, PM>(dv<R, PM>(ux(2), gx, imax, inv_dm), mx);
+        const R uxv = met<R, PM>(dv<R, PM>(ux(1), gx, imax, inv_dm, A.xper), mx);
+        const R vxv = met<R, PM>(dv<R, PM>(ux(2), gx, imax, inv_dm, A.xper), mx);
         R uyv = met<R, PM>(dv<R, PM>(uy(1), gy, jmax, inv_dn), my);
         R vyv = met<R, PM>(dv<R, PM>(uy(2), gy, jmax, inv_dn), my);
-        const R Tx = met<R, PM>(dv<R, PM>(tx, gx, imax, inv_dm), mx);
+        const R Tx = met<R, PM>(dv<R, PM>(tx, gx, imax, inv_dm, A.xper), mx);
         R Ty = met<R, PM>(dv<R, PM>(ty, gy, jmax, inv_dn), my);
         if constexpr (PERT) {
             uyv = uyv + (rc ? rc->dub[1] : R(A.pr.dub[1][gy]));
@@ -179,7 +238,7 @@ __device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy,
     for (int k = 0; k < NK; ++k) {
         qc[k] = ux(k)(gx);
         uc[k] = qc[k];
-        fx[k] = met<R, PM>(dv<R, PM>(ux(k), gx, imax, inv_dm), mx);
+        fx[k] = met<R, PM>(dv<R, PM>(ux(k), gx, imax, inv_dm, A.xper), mx);
         fy[k] = met<R, PM>(dv<R, PM>(uy(k), gy, jmax, inv_dn), my);
         if constexpr (PERT) {
             uc[k] = qc[k] + (rc ? rc->ub[k] : R(A.pr.ub[k][gy]));
@@ -192,9 +251,9 @@ __device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy,
     // eqeuler.f90:63-66
     f[0] = -(u * fx[0] + rho * fx[1] + v * fy[0] + rho * fy[2]);
     if constexpr (VISC) {
-        const R txx_x = met<R, PM>(dv<R, PM>(lx(0), gx, imax, inv_dm), mx);
-        const R txy_x = met<R, PM>(dv<R, PM>(lx(1), gx, imax, inv_dm), mx);
-        const R ktx_x = met<R, PM>(dv<R, PM>(lx(3), gx, imax, inv_dm), mx);
+        const R txx_x = met<R, PM>(dv<R, PM>(lx(0), gx, imax, inv_dm, A.xper), mx);
+        const R txy_x = met<R, PM>(dv<R, PM>(lx(1), gx, imax, inv_dm, A.xper), mx);
+        const R ktx_x = met<R, PM>(dv<R, PM>(lx(3), gx, imax, inv_dm, A.xper), mx);
         const R txy_y = met<R, PM>(dv<R, PM>(ly(1), gy, jmax, inv_dn), my);
         const R tyy_y = met<R, PM>(dv<R, PM>(ly(2), gy, jmax, inv_dn), my);
         const R kty_y = met<R, PM>(dv<R, PM>(ly(4), gy, jmax, inv_dn), my);
@@ -213,7 +272,7 @@ __device__ __forceinline__ void rhs_point(const StageArgs<S> &A, int gx, int gy,
     }
     if constexpr (NK >= 5) {
         constexpr int P = L1<NK, VISC>::PSI;
-        const R psd_x = met<R, PM>(dv<R, PM>(lx(P), gx, imax, inv_dm), mx);
+        const R psd_x = met<R, PM>(dv<R, PM>(lx(P), gx, imax, inv_dm, A.xper), mx);
         const R psd_y = met<R, PM>(dv<R, PM>(ly(P), gy, jmax, inv_dn), my);
         f[4] = -(u * fx[4] + v * fy[4] - R(A.inv_Pe) * irho * (psd_x + psd_y));  // eqeuler.f90:94-95
     }
@@ -241,7 +300,7 @@ __device__ __forceinline__ void store_prefetch(const StageArgs<S> &A, int gx, in
     }
     if constexpr (PM != PM_FAST) {
         const bool ybuf = gy < A.D || gy >= A.jmax - A.D;
-        const bool xbuf = gx < A.D || gx >= A.imax - A.D;
+        const bool xbuf = !A.xper && (gx < A.D || gx >= A.imax - A.D);
         p.sy = ybuf ? A.sigy[gy] : 1.0;
         p.sx = xbuf ? A.sigx[gx] : 1.0;
 #pragma unroll
@@ -263,9 +322,11 @@ __device__ __forceinline__ void stage_store(const StageArgs<S> &A, int gx, int g
         for (int k = 0; k < NK; ++k) A.dudt[k][g] = S(f[k]);
         return;
     }
-    const bool interior = PM != PM_GEN || (gx >= 1 && gx <= imax - 2 && gy >= 1 && gy <= jmax - 2);
+    // interior points (rk.f90:161-162; periodic x: every column)
+    const bool interior =
+        PM != PM_GEN || ((A.xper || (gx >= 1 && gx <= imax - 2)) && gy >= 1 && gy <= jmax - 2);
     const bool ybuf = PM != PM_FAST && (gy < A.D || gy >= jmax - A.D);
-    const bool xbuf = PM != PM_FAST && (gx < A.D || gx >= imax - A.D);
+    const bool xbuf = PM != PM_FAST && !A.xper && (gx < A.D || gx >= imax - A.D);
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
         if constexpr (PERT) {
@@ -317,7 +378,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1
     R *sL = sT + (VISC ? WY * WX : 0);        // level-1 fields: LY x LX each
 
     const int imax = A.imax, jmax = A.jmax, pitch = A.pitch;
-    const Range rx = tile_ranges(i0, i1, imax);
+    // periodic x: the windows are not shifted near the x ends; their columns
+    // wrap when loaded
+    const Range rx = A.xper ? Range{i0 - 3, i1 + 3, i0 - 6, i1 + 6} : tile_ranges(i0, i1, imax);
     const Range ry = tile_ranges(j0, j1, jmax);
     const int tid = threadIdx.x;
     const R gam = R(A.gamma);
@@ -327,7 +390,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1
     for (int idx = tid; idx < wyn * WX; idx += NT) {
         const int wy = idx / WX, wx = idx - wy * WX;
         if (wx < wxn) {
-            const size_t g = (size_t)(ry.w0 + wy) * pitch + (rx.w0 + wx);
+            const size_t g = (size_t)(ry.w0 + wy) * pitch + wrap_col(rx.w0 + wx, imax);
 #pragma unroll
             for (int k = 0; k < NK; ++k) sU[(k * WY + wy) * WX + wx] = R(A.U[k][g]);
             if constexpr (VISC) {
@@ -373,7 +436,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1
             const R *tcol = sT + (gx - ox);
             R out[NL];
             level1_point<S, R, NK, VISC, PERT, PM>(
-                A, tc, gx, gy, PM == PM_FAST ? R(1) : R(A.metx[gx]), PM == PM_FAST ? R(1) : R(A.mety[gy]),
+                A, tc, gx, gy, PM == PM_FAST ? R(1) : R(A.metx[wrap_col(gx, imax)]),
+                PM == PM_FAST ? R(1) : R(A.mety[gy]),
                 ux_at(gy), uy_at(gx), [trow, ox](int q) { return trow[q - ox]; },
                 [tcol, oy](int q) { return tcol[(q - oy) * WX]; }, VISC ? trow[gx - ox] : R(0), out);
             const int o = ly * LX + lx;
@@ -518,10 +582,12 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
     const int wy0 = y0 - 6, ly0 = y0 - 3;  // first window / level-1 row of the chunk
     const R gam = R(A.gamma);
 
-    // window columns past the grid (partial last strip) load the last column
+    // window columns past the grid (partial last strip) load the last column;
+    // periodic x: every window column wraps around the period
     const int cmax = A.imax - 1 - ox;
+    const bool xper = A.xper;
     auto load_pt = [&](int r, int c, R *v) {
-        const size_t g = (size_t)r * pitch + (ox + (c < cmax ? c : cmax));
+        const size_t g = (size_t)r * pitch + (xper ? wrap_col(ox + c, A.imax) : ox + (c < cmax ? c : cmax));
 #pragma unroll
         for (int k = 0; k < NK; ++k) v[k] = R(A.U[k][g]);
     };
@@ -571,8 +637,10 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
     // metric factors of this lane's level-1 and output columns (fixed for
     // the whole march)
     constexpr bool MET = PM != PM_FAST;
-    const R mxl = MET ? R(A.metx[min(lx0 + lane, A.imax - 1)]) : R(1);
-    const R mxo = MET ? R(A.metx[min(i0 + lane, A.imax - 1)]) : R(1);
+    const int cl = xper ? wrap_col(lx0 + lane, A.imax) : min(lx0 + lane, A.imax - 1);
+    const int co = xper ? wrap_col(i0 + lane, A.imax) : min(i0 + lane, A.imax - 1);
+    const R mxl = MET ? R(A.metx[cl]) : R(1);
+    const R mxo = MET ? R(A.metx[co]) : R(1);
     // wave-uniform row r, column lx0 + lane; my: mety[r]
     auto level1_row = [&](int r, const RowCache<R, NK> *rc, R my) {
         const int gx = lx0 + lane;
@@ -699,33 +767,29 @@ static size_t stage_smem(int NK, bool visc, int TY, int TX = STX)
 // The marching region: output columns [xa, xa + 58 n) and rows [ya, yb),
 // rows a multiple of RB (march_region, march_hybrid); chunk height H from a
 // simple occupancy model (blocks per wave of the GPU x rows per block).
+// False when there is no such region (e is then hipSuccess) or when the
+// marching kernel could not be set up (e says why).
 template <typename S, int NK, bool VISC, bool PERT, int RB, int PM>
-static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks, bool &one_wave)
+static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks, bool &one_wave, hipStream_t s,
+                       hipError_t &e)
 {
     constexpr bool FAST = PM == PM_FAST;
+    e = hipSuccess;
     if ((FAST || PM == PM_HYBRID) && !A.fast_ok) return false;
     if constexpr (PM == PM_HYBRID) {
-        if (!march_hybrid(A.D, A.imax, A.jmax, RB, m)) return false;
+        if (!march_hybrid(A.D, A.imax, A.jmax, RB, m, A.xper)) return false;
     } else {
-        if (!march_region(FAST, A.D, A.imax, A.jmax, RB, m)) return false;
+        if (!march_region(FAST, A.D, A.imax, A.jmax, RB, m, A.xper)) return false;
     }
     // chunks of H rows over [y_a, y_b); the edge parts are one chunk each
     // (march_chunks, march_chunk_rows)
     const int rows = m.y_b - m.y_a;
     const int edges = (m.y_a > m.ya ? 1 : 0) + (m.yb > m.y_b ? 1 : 0);
-    static int ncu = 0, perblk = 0;
-    if (!ncu) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        if (ncu <= 0) ncu = 256;
-        auto kern = stage_march_kernel<S, S, NK, VISC, PERT, RB, PM>;
-        const size_t smem = march_smem<S>(NK, VISC, RB);
-        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perblk, kern, RB * 64, smem) != hipSuccess || perblk < 1)
-            perblk = 1;
-    }
-    const long slots = (long)ncu * perblk;
+    KernelSetup ks;
+    if ((e = kernel_ready((const void *)stage_march_kernel<S, S, NK, VISC, PERT, RB, PM>, march_smem<S>(NK, VISC, RB),
+                          RB * 64, s, &ks)) != hipSuccess)
+        return false;
+    const long slots = (long)ks.ncu * ks.perblk;
     long best = -1;
     for (int nc = 1; nc <= rows / (4 * RB); ++nc) {
         const int H = ((rows + nc - 1) / nc + RB - 1) / RB * RB;
@@ -765,19 +829,22 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         // unit spacing drops the 1/h factors)
         const int gpm = (A.inv_dm == 1.0 && A.inv_dn == 1.0) ? PM_MIDU : PM_MID;
         pm = gpm;
+        hipError_t pe = hipSuccess;
         if (A.fast_ok) {
             if constexpr (GENERAL) {
                 pm = PM_HYBRID;
-                inner = march_plan<S, NK, VISC, PERT, RB, PM_HYBRID>(A, m, nchunks, one_wave);
+                inner = march_plan<S, NK, VISC, PERT, RB, PM_HYBRID>(A, m, nchunks, one_wave, s, pe);
             } else {
                 pm = PM_FAST;
-                inner = march_plan<S, NK, VISC, PERT, RB, PM_FAST>(A, m, nchunks, one_wave);
+                inner = march_plan<S, NK, VISC, PERT, RB, PM_FAST>(A, m, nchunks, one_wave, s, pe);
             }
         }
+        if (pe != hipSuccess) return pe;
         if (!inner) {
             pm = gpm;
-            inner = pm == PM_MIDU ? march_plan<S, NK, VISC, PERT, RB, PM_MIDU>(A, m, nchunks, one_wave)
-                                  : march_plan<S, NK, VISC, PERT, RB, PM_MID>(A, m, nchunks, one_wave);
+            inner = pm == PM_MIDU ? march_plan<S, NK, VISC, PERT, RB, PM_MIDU>(A, m, nchunks, one_wave, s, pe)
+                                  : march_plan<S, NK, VISC, PERT, RB, PM_MID>(A, m, nchunks, one_wave, s, pe);
+            if (pe != hipSuccess) return pe;
         }
         if (inner) xa = m.xa, xb = m.xe, ya = m.ya, yb = m.yb;
     } else {
@@ -814,9 +881,7 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         if (small) gen = stage_kernel_edges<S, R, NK, VISC, TY, RHS_ONLY, PERT, true>;
     }
     const int ent = small ? ES::NT : EL::NT;
-    if ((e = hipFuncSetAttribute((const void *)gen, hipFuncAttributeMaxDynamicSharedMemorySize, (int)esm)) !=
-        hipSuccess)
-        return e;
+    if ((e = kernel_ready((const void *)gen, esm, ent, s)) != hipSuccess) return e;
     // An edge tile needs most of a CU's LDS, so it cannot share a CU with
     // interior blocks.  The interior kernel is launched first; the edge
     // tiles, launched after it on s2, take CUs as the interior blocks drain
@@ -848,12 +913,10 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
             if (pm == PM_FAST) march = stage_march_kernel<S, R, NK, VISC, PERT, RB, PM_FAST>;
         }
         msm = march_smem<R>(NK, VISC, RB);
-        if (inner && (e = hipFuncSetAttribute((const void *)march, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)msm)) != hipSuccess)
-            return e;
+        if (inner && (e = kernel_ready((const void *)march, msm, RB * 64, s)) != hipSuccess) return e;
     } else if constexpr (!RHS_ONLY) {
-        if (inner && (e = hipFuncSetAttribute((const void *)stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) != hipSuccess)
+        if (inner && (e = kernel_ready((const void *)stage_kernel_fast<S, R, NK, VISC, TY, PERT, NT>, smem, NT, s)) !=
+                         hipSuccess)
             return e;
     }
     if (tm && tm->plan_only) return hipSuccess;
@@ -930,7 +993,7 @@ static hipError_t launch_stage_batch_t(const StageArgs<S> *args, const int4 *til
     if (ntiles <= 0) return hipSuccess;
     size_t smem = stage_smem<R>(NK, VISC, TY);
     auto kern = stage_kernel_batch<S, R, NK, VISC, TY, PERT, NT, FAST>;
-    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    hipError_t e = kernel_ready((const void *)kern, smem, NT, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(ntiles), dim3(NT), smem, s, args, tiles, stage, ncases);
     return hipGetLastError();
@@ -1051,18 +1114,33 @@ struct LineEnds {
     double vn2, vn3;    // final values at positions n-2, n-3 (last chunk)
 };
 
-// IO policy contract: load(p) for strictly increasing p (p < n),
-// emit(e, v) for strictly increasing e, finish() at the end.  PD: groups of
-// 13 loads issued ahead of use (2 for direct global loads, so that a load has
-// about 26 steps of work to arrive; 1 where the IO prefetches itself).
-template <typename R, bool YDIR, int PD, typename IO>
+// IO policy contract: load(p) for strictly increasing p (p < n; RING: p may
+// start at -6, which reads the unfiltered point n + p), emit(e, v) for
+// strictly increasing e (positions outside [a, b) are dropped), finish() at
+// the end; RING also needs load_head(p), p in [0, 12), a direct read of the
+// unfiltered point p.  PD: groups of 13 loads issued ahead of use (2 for
+// direct global loads, so that a load has about 26 steps of work to arrive;
+// 1 where the IO prefetches itself).
+//
+// RING (periodic x, no reference counterpart): the interior update sweeps
+// around the ring from position 0: the exact-start chunk begins at q = 0
+// with the unfiltered points n-6..n-1 on its left; the chunk holding the
+// end updates positions n-6..n-1 last, reading on its right the points 0..5
+// as the start of the sweep updated them, which it recomputes from the
+// unfiltered points n-6..n-1 and 0..11 (the same operations, so the same
+// bits).
+template <typename R, bool YDIR, int PD, typename IO, bool RING = false>
 __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b, int W, LineEnds &ends)
 {
     const bool last = (b == n);
     const bool exact = (a <= W + 12);
     R w[13];
     int q;
-    if (exact) {
+    if (RING && exact) {
+#pragma unroll
+        for (int s = 0; s < 13; ++s) w[s] = io.load(s - 6);
+        q = 0;
+    } else if (exact) {
 #pragma unroll
         for (int s = 0; s < 13; ++s) w[s] = io.load(s);
         // left block, ascending (filtering.f90:36-49)
@@ -1082,6 +1160,8 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
         for (int s = 0; s < 13; ++s) w[s] = io.load(q0 - 6 + s);
         q = q0;
     }
+    // RING: the chunk holding the end stops its loop at n - 7 as well, and
+    // updates n-6..n-1 after it against the head
     const int qe = last ? n - 7 : b - 1;
     // loads for the steps of the next PD groups of 13 are in flight.  Every
     // load is issued, at a clamped position: a value fetched past the line
@@ -1134,7 +1214,39 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
     }
     R v[13];  // positions qe-6 .. qe+6
     unrotate<R>(rot, w, v);
-    if (last) {
+    if (RING && last) {
+        // the head: positions 0..5 as the sweep's start updated them, from
+        // the unfiltered positions n-6..n-1 (v[7..12], not yet updated here)
+        // and 0..11
+        R hv[18];  // positions -6 .. 11
+#pragma unroll
+        for (int s = 0; s < 6; ++s) hv[s] = v[7 + s];
+#pragma unroll
+        for (int s = 0; s < 12; ++s) hv[6 + s] = io.load_head(s);
+#pragma unroll
+        for (int qq = 0; qq < 6; ++qq) {
+            R Du = R(filt_d(0)) * hv[qq];
+#pragma unroll
+            for (int s = 1; s < 13; ++s) Du = Du + R(filt_d(s)) * hv[qq + s];
+            hv[qq + 6] = hv[qq + 6] - sigma * Du;
+        }
+        // positions n-6 .. n-1 in order; ext[s]: position n-13+s, the
+        // points past the end are the updated head
+        R ext[19];
+#pragma unroll
+        for (int s = 0; s < 13; ++s) ext[s] = v[s];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) ext[13 + s] = hv[6 + s];
+#pragma unroll
+        for (int qq = 7; qq <= 12; ++qq) {
+            R Du = R(filt_d(0)) * ext[qq - 6];
+#pragma unroll
+            for (int s = 1; s < 13; ++s) Du = Du + R(filt_d(s)) * ext[qq - 6 + s];
+            ext[qq] = ext[qq] - sigma * Du;
+        }
+#pragma unroll
+        for (int s = 0; s < 13; ++s) io.emit(qe - 6 + s, ext[s]);
+    } else if (last) {
         // right block, descending, mirrored stencil (filtering.f90:81-94)
 #pragma unroll
         for (int qq = 11; qq >= 6; --qq) {
@@ -1166,6 +1278,7 @@ struct XIO {
     const R *in;
     R *out;
     int pitch, n, m, row0, a, b, lane;
+    bool ring;  // periodic line: positions below 0 wrap to n + p
     R (*tin)[T + 1];
     R (*tout)[T + 1];
     int tc;  // start of the tile in LDS
@@ -1178,7 +1291,7 @@ struct XIO {
         // tile slots that are never read
         const int c = lane % T, rs = lane / T;
         int gi = t + c;
-        gi = gi < 0 ? 0 : (gi < n ? gi : n - 1);
+        gi = gi < 0 ? (ring ? gi + n : 0) : (gi < n ? gi : n - 1);
 #pragma unroll
         for (int rr = 0; rr < T; ++rr) {
             int gj = row0 + rr * RP + rs;
@@ -1233,6 +1346,11 @@ struct XIO {
         tout[lane][e - oc] = v;
     }
     __device__ __forceinline__ void finish() { flush(); }
+    __device__ __forceinline__ R load_head(int p)  // this lane's row, position p (unstaged)
+    {
+        const int gj = row0 + lane < m ? row0 + lane : m - 1;
+        return in[(size_t)gj * pitch + p];
+    }
 };
 
 template <typename R>
@@ -1255,10 +1373,16 @@ __device__ __forceinline__ void filter_x_tile(const FilterArgs<R> &F, int chunk,
     io.lane = threadIdx.x;
     io.tin = tin;
     io.tout = tout;
+    io.ring = F.xper;
     const bool exact = (a <= F.W + 12);
-    io.start(exact ? 0 : a - F.W - 6);
     LineEnds ends;
-    filter_line<R, false, 1>(io, R(F.sigma), F.n, a, b, F.W, ends);
+    if (F.xper) {
+        io.start(exact ? -6 : a - F.W - 6);
+        filter_line<R, false, 1, XIO<R>, true>(io, R(F.sigma), F.n, a, b, F.W, ends);
+    } else {
+        io.start(exact ? 0 : a - F.W - 6);
+        filter_line<R, false, 1>(io, R(F.sigma), F.n, a, b, F.W, ends);
+    }
 }
 
 template <typename R>
@@ -1292,6 +1416,7 @@ struct YIO {
         if (active && e >= a && e < b) out[(size_t)e * pitch + col] = corr ? R(double(v) + corr[e]) : v;
     }
     __device__ __forceinline__ void finish() {}
+    __device__ __forceinline__ R load_head(int p) { return in[(size_t)p * pitch + colc]; }
 };
 
 template <typename R>
@@ -1312,8 +1437,9 @@ __device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblo
     io.active = col < F.m;
     LineEnds ends;
     filter_line<R, true, 2>(io, R(F.sigma), F.n, a, b, F.W, ends);
-    // ccpml rows j=1 and j=jmax for columns 2..imax-1 (cc.f90:46-59)
-    if (F.edge_bc && io.active && col >= 1 && col <= F.m - 2) {
+    // ccpml rows j=1 and j=jmax for columns 2..imax-1 (cc.f90:46-59; periodic
+    // x: every column, there are no inflow / outflow columns)
+    if (F.edge_bc && io.active && (F.xper || (col >= 1 && col <= F.m - 2))) {
         R *o = F.out[k];
         const int jm = F.n;
         const double *cr = F.corr[k];
@@ -1361,6 +1487,8 @@ template <typename S>
 static hipError_t launch_filter_t(const FilterArgs<S> &F, bool xdir, hipStream_t s)
 {
     const int nchunks = (F.n + F.L - 1) / F.L;
+    hipError_t e = kernel_ready(xdir ? (const void *)filter_x_kernel<S> : (const void *)filter_y_kernel<S>, 0, 64, s);
+    if (e != hipSuccess) return e;
     if (xdir) {
         dim3 grid(nchunks, (F.m + 63) / 64, F.nk);
         hipLaunchKernelGGL(filter_x_kernel<S>, grid, dim3(64), 0, s, F);
@@ -1375,6 +1503,9 @@ template <typename S>
 static hipError_t launch_filter_batch_t(const FilterArgs<S> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
                                         hipStream_t s)
 {
+    hipError_t e = kernel_ready(
+        xdir ? (const void *)filter_x_kernel_batch<S> : (const void *)filter_y_kernel_batch<S>, 0, 64, s);
+    if (e != hipSuccess) return e;
     if (xdir)
         hipLaunchKernelGGL(filter_x_kernel_batch<S>, dim3(ntiles, nk), dim3(64), 0, s, args, tiles);
     else
@@ -1406,7 +1537,7 @@ hipError_t launch_filter_batch(const FilterArgs<float> *args, const int4 *tiles,
 template <typename S>
 __device__ __forceinline__ void edge_i_point(const EdgeArgs<S> &E, int j, int side)
 {
-    if (j >= E.jmax) return;
+    if (j >= E.jmax || E.xper) return;
     const int imax = E.imax, jmax = E.jmax, pitch = E.pitch;
     const double ig = 1.0 / E.gamma;
     auto at = [&](int k, int i, int jj) { return double(E.U[k][(size_t)jj * pitch + i]); };
@@ -1460,6 +1591,8 @@ __global__ void edge_i_kernel_batch(const EdgeArgs<S> *args)
 template <typename S>
 static hipError_t launch_edge_t(const EdgeArgs<S> &E, hipStream_t s)
 {
+    hipError_t e = kernel_ready((const void *)edge_i_kernel<S>, 0, 256, s);
+    if (e != hipSuccess) return e;
     dim3 grid((E.jmax + 255) / 256, 2);
     hipLaunchKernelGGL(edge_i_kernel<S>, grid, dim3(256), 0, s, E);
     return hipGetLastError();
@@ -1468,6 +1601,8 @@ static hipError_t launch_edge_t(const EdgeArgs<S> &E, hipStream_t s)
 template <typename S>
 static hipError_t launch_edge_batch_t(const EdgeArgs<S> *args, int ncases, hipStream_t s)
 {
+    hipError_t e = kernel_ready((const void *)edge_i_kernel_batch<S>, 0, 256, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(edge_i_kernel_batch<S>, dim3(ncases, 2), dim3(256), 0, s, args);
     return hipGetLastError();
 }
@@ -1512,10 +1647,9 @@ static hipError_t prepare_step_t(const StageArgs<S> &A, int nk, bool visc, int v
         kerns[1] = (const void *)filter_y_kernel<S>;
         kerns[2] = (const void *)edge_i_kernel<S>;
     }
-    for (const void *k : kerns) {
-        hipFuncAttributes fa;
-        if ((e = hipFuncGetAttributes(&fa, k)) != hipSuccess) return e;
-    }
+    const int threads[3] = {64, 64, 256};
+    for (int q = 0; q < 3; ++q)
+        if ((e = kernel_ready(kerns[q], 0, threads[q], nullptr)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -1538,7 +1672,7 @@ hipError_t prepare_step(const StageArgs<float> &A, int nk, bool visc, int varian
 // Operator-level kernels for the parity API (one thread per point).
 // ---------------------------------------------------------------------------
 __global__ void deriv_kernel(const double *const *in, double *const *out, int nk, int imax, int jmax, int pitch,
-                             int ydir, const double *met, double inv_h)
+                             int ydir, const double *met, double inv_h, int xper)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= imax) return;
@@ -1547,7 +1681,10 @@ __global__ void deriv_kernel(const double *const *in, double *const *out, int nk
         double d;
         if (!ydir) {
             const double *row = u + (size_t)j * pitch;
-            d = drp_deriv<double>([row](int q) { return row[q]; }, i, imax, inv_h) * met[i];
+            if (xper)  // periodic x: the interior stencil, wrapped
+                d = drp_interior<double>([row, imax](int q) { return row[wrap_col(q, imax)]; }, i, inv_h) * met[i];
+            else
+                d = drp_deriv<double>([row](int q) { return row[q]; }, i, imax, inv_h) * met[i];
         } else {
             const double *col = u + i;
             d = drp_deriv<double>([col, pitch](int q) { return col[(size_t)q * pitch]; }, j, jmax, inv_h) * met[j];
@@ -1557,10 +1694,11 @@ __global__ void deriv_kernel(const double *const *in, double *const *out, int nk
 }
 
 hipError_t launch_deriv(const double *const *in, double *const *out, int nk, int imax, int jmax, int pitch, bool ydir,
-                        const double *met, double inv_h, hipStream_t s)
+                        const double *met, double inv_h, bool xper, hipStream_t s)
 {
     dim3 grid((imax + 127) / 128, jmax);
-    hipLaunchKernelGGL(deriv_kernel, grid, dim3(128), 0, s, in, out, nk, imax, jmax, pitch, (int)ydir, met, inv_h);
+    hipLaunchKernelGGL(deriv_kernel, grid, dim3(128), 0, s, in, out, nk, imax, jmax, pitch, (int)ydir, met, inv_h,
+                       (int)xper);
     return hipGetLastError();
 }
 
@@ -1583,10 +1721,10 @@ hipError_t launch_transport(const double *T, double *MI, double *K1, size_t n, c
 
 // rk_steep on nk planes (rk.f90:160-167)
 __global__ void rk_steep_kernel(double *const *U, const double *const *dudt, double *const *W, int nk, int imax,
-                                int jmax, int pitch, double alpha, double beta, double dt)
+                                int jmax, int pitch, double alpha, double beta, double dt, int xper)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i < 1 || i > imax - 2 || j < 1 || j > jmax - 2) return;
+    if (i >= imax || (!xper && (i < 1 || i > imax - 2)) || j < 1 || j > jmax - 2) return;
     const size_t g = (size_t)j * pitch + i;
     for (int k = 0; k < nk; ++k) {
         const double wn = alpha * W[k][g] + dt * dudt[k][g];
@@ -1596,20 +1734,21 @@ __global__ void rk_steep_kernel(double *const *U, const double *const *dudt, dou
 }
 
 hipError_t launch_rk_steep(double *const *U, const double *const *dudt, double *const *W, int nk, int imax, int jmax,
-                           int pitch, double alpha, double beta, double dt, hipStream_t s)
+                           int pitch, double alpha, double beta, double dt, bool xper, hipStream_t s)
 {
     dim3 grid((imax + 127) / 128, jmax);
-    hipLaunchKernelGGL(rk_steep_kernel, grid, dim3(128), 0, s, U, dudt, W, nk, imax, jmax, pitch, alpha, beta, dt);
+    hipLaunchKernelGGL(rk_steep_kernel, grid, dim3(128), 0, s, U, dudt, W, nk, imax, jmax, pitch, alpha, beta, dt,
+                       (int)xper);
     return hipGetLastError();
 }
 
 // bufferxy (rk.f90:77-151)
 __global__ void buffer_kernel(double *const *U, const double *const *base, int nk, int imax, int jmax, int pitch,
-                              int D, const double *sigx, const double *sigy)
+                              int D, const double *sigx, const double *sigy, int xper)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= imax) return;
-    const bool ybuf = j < D || j >= jmax - D, xbuf = i < D || i >= imax - D;
+    const bool ybuf = j < D || j >= jmax - D, xbuf = !xper && (i < D || i >= imax - D);
     if (!ybuf && !xbuf) return;
     const size_t g = (size_t)j * pitch + i;
     for (int k = 0; k < nk; ++k) {
@@ -1622,10 +1761,11 @@ __global__ void buffer_kernel(double *const *U, const double *const *base, int n
 }
 
 hipError_t launch_buffer(double *const *U, const double *const *base, int nk, int imax, int jmax, int pitch, int D,
-                         const double *sigx, const double *sigy, hipStream_t s)
+                         const double *sigx, const double *sigy, bool xper, hipStream_t s)
 {
     dim3 grid((imax + 127) / 128, jmax);
-    hipLaunchKernelGGL(buffer_kernel, grid, dim3(128), 0, s, U, base, nk, imax, jmax, pitch, D, sigx, sigy);
+    hipLaunchKernelGGL(buffer_kernel, grid, dim3(128), 0, s, U, base, nk, imax, jmax, pitch, D, sigx, sigy,
+                       (int)xper);
     return hipGetLastError();
 }
 
@@ -1642,6 +1782,119 @@ hipError_t launch_source(double *s_out, int imax, int jmax, double amp, const do
 {
     dim3 grid((imax + 127) / 128, jmax);
     hipLaunchKernelGGL(source_kernel, grid, dim3(128), 0, s, s_out, imax, jmax, amp, gx, gy);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Pieces of rhs_euler as the reference splits them (derivs.f90, eqeuler.f90,
+// diff.f90 metricsX/Y), one thread per point on pitched planes; they serve
+// the drop-in modules of the same names.  Each keeps the reference's
+// expression order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool pw_point(const PwArgs &a, size_t &g, int &i, int &j)
+{
+    i = blockIdx.x * blockDim.x + threadIdx.x;
+    j = blockIdx.y;
+    g = (size_t)j * a.pitch + i;
+    return i < a.imax;
+}
+
+// kdT (derivs.f90:48-56): dTdm *= k, dTdn *= k
+__global__ void kdt_kernel(PwArgs a, const double *k, double *dTdm, double *dTdn)
+{
+    size_t g;
+    int i, j;
+    if (!pw_point(a, g, i, j)) return;
+    dTdn[g] = dTdn[g] * k[g];
+    dTdm[g] = dTdm[g] * k[g];
+}
+
+// psiDiff = rho alpha (dpsi/dm + dpsi/dn) (derivs.f90:36-40)
+__global__ void psidiff_kernel(PwArgs a, const double *rho, const double *pm, const double *pn, double alpha,
+                               double *out)
+{
+    size_t g;
+    int i, j;
+    if (!pw_point(a, g, i, j)) return;
+    out[g] = rho[g] * alpha * (pm[g] + pn[g]);
+}
+
+// tau_ij (eqeuler.f90:34-41) from MI and the u, v derivatives
+__global__ void tau_kernel(PwArgs a, const double *mi, const double *ux, const double *uy, const double *vx,
+                           const double *vy, double *txx, double *txy, double *tyy)
+{
+    size_t g;
+    int i, j;
+    if (!pw_point(a, g, i, j)) return;
+    const double m = mi[g];
+    txx[g] = -(2.0 / 3.0 * m * (ux[g] + vy[g])) + 2.0 * m * ux[g];
+    txy[g] = m * (uy[g] + vx[g]);
+    tyy[g] = -(2.0 / 3.0 * m * (ux[g] + vy[g])) + 2.0 * m * vy[g];
+}
+
+// EULERNONLINEAR (eqeuler.f90:61-97)
+__global__ void euler_pw_kernel(EulerPwArgs e)
+{
+    size_t g;
+    int i, j;
+    if (!pw_point(e.a, g, i, j)) return;
+    const double u0 = e.U[0][g], u1 = e.U[1][g], u2 = e.U[2][g], u3 = e.U[3][g];
+    const double gamma = e.gamma, Re = e.Re, Pr = e.Pr;
+    const double s = e.src ? e.src[(size_t)j * e.a.imax + i] : 0.0;
+    auto dm = [&](int k) { return e.Um[k][g]; };
+    auto dn = [&](int k) { return e.Un[k][g]; };
+    e.out[0][g] = -(u1 * dm(0) + u0 * dm(1) + u2 * dn(0) + u0 * dn(2));
+    e.out[1][g] = -(u1 * dm(1) + 1.0 / u0 * dm(3) + u2 * dn(1) - 1.0 / Re / u0 * (e.taum[0][g] + e.taun[1][g]));
+    e.out[2][g] = -(u1 * dm(2) + u2 * dn(2) + 1.0 / u0 * dn(3) - 1.0 / Re / u0 * (e.taum[1][g] + e.taun[2][g]));
+    e.out[3][g] = -(gamma * u3 * dm(1) + u1 * dm(3) + gamma * u3 * dn(2) + u2 * dn(3) - s -
+                    (gamma - 1.0) / Re *
+                        (+dm(1) * e.tau[0][g] + dm(2) * e.tau[1][g] + dn(1) * e.tau[1][g] + dn(2) * e.tau[2][g]) -
+                    1.0 / Re / Pr * (e.d2Tm[g] + e.d2Tn[g]));
+    if (e.nk >= 5)
+        e.out[4][g] = -(u1 * dm(4) + u2 * dn(4) - 1.0 / e.Pe / u0 * (e.d2pm[g] + e.d2pn[g]));
+}
+
+// metricsX / metricsY (diff.f90:276-350): du *= the point's effective factor
+__global__ void metric_kernel(PwArgs a, double *const *dU, int nk, const double *met, int ydir)
+{
+    size_t g;
+    int i, j;
+    if (!pw_point(a, g, i, j)) return;
+    const double f = ydir ? met[j] : met[i];
+    for (int k = 0; k < nk; ++k) dU[k][g] = dU[k][g] * f;
+}
+
+static dim3 pw_grid(const PwArgs &a) { return dim3((a.imax + 127) / 128, a.jmax); }
+
+hipError_t launch_kdt(const PwArgs &a, const double *k, double *dTdm, double *dTdn, hipStream_t s)
+{
+    hipLaunchKernelGGL(kdt_kernel, pw_grid(a), dim3(128), 0, s, a, k, dTdm, dTdn);
+    return hipGetLastError();
+}
+
+hipError_t launch_psidiff(const PwArgs &a, const double *rho, const double *pm, const double *pn, double alpha,
+                          double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(psidiff_kernel, pw_grid(a), dim3(128), 0, s, a, rho, pm, pn, alpha, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_tau(const PwArgs &a, const double *mi, const double *ux, const double *uy, const double *vx,
+                      const double *vy, double *txx, double *txy, double *tyy, hipStream_t s)
+{
+    hipLaunchKernelGGL(tau_kernel, pw_grid(a), dim3(128), 0, s, a, mi, ux, uy, vx, vy, txx, txy, tyy);
+    return hipGetLastError();
+}
+
+hipError_t launch_euler_pw(const EulerPwArgs &e, hipStream_t s)
+{
+    hipLaunchKernelGGL(euler_pw_kernel, pw_grid(e.a), dim3(128), 0, s, e);
+    return hipGetLastError();
+}
+
+hipError_t launch_metric(const PwArgs &a, double *const *dU, int nk, const double *met, bool ydir, hipStream_t s)
+{
+    hipLaunchKernelGGL(metric_kernel, pw_grid(a), dim3(128), 0, s, a, dU, nk, met, (int)ydir);
     return hipGetLastError();
 }
 
@@ -1678,14 +1931,16 @@ template hipError_t launch_probe<float>(const float *const *, int, int, int, con
 // row with the same operator.  out: jmax x imax, row-major.
 template <typename S>
 __global__ __launch_bounds__(256) void vorticity_kernel(const S *u, const S *v, const double *ub1, int imax, int jmax,
-                                                        int pitch, double inv_dm, double inv_dn, double *out)
+                                                        int pitch, double inv_dm, double inv_dn, int xper, double *out)
 {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= imax || j >= jmax) return;
     const S *vrow = v + (size_t)j * pitch;
     const S *ucol = u + i;
-    const double dvdm = drp_deriv<double>([vrow](int q) { return double(vrow[q]); }, i, imax, inv_dm);
+    const double dvdm =
+        xper ? drp_interior<double>([vrow, imax](int q) { return double(vrow[wrap_col(q, imax)]); }, i, inv_dm)
+             : drp_deriv<double>([vrow](int q) { return double(vrow[q]); }, i, imax, inv_dm);
     double dudn = drp_deriv<double>([ucol, pitch](int q) { return double(ucol[(size_t)q * pitch]); }, j, jmax, inv_dn);
     if (ub1) dudn = dudn + drp_deriv<double>([ub1](int q) { return ub1[q]; }, j, jmax, inv_dn);
     out[(size_t)j * imax + i] = dvdm - dudn;
@@ -1693,19 +1948,19 @@ __global__ __launch_bounds__(256) void vorticity_kernel(const S *u, const S *v, 
 
 template <typename S>
 hipError_t launch_vorticity(const S *u, const S *v, const double *ub1, int imax, int jmax, int pitch, double inv_dm,
-                            double inv_dn, double *out, hipStream_t s)
+                            double inv_dn, bool xper, double *out, hipStream_t s)
 {
     hipLaunchKernelGGL(vorticity_kernel<S>, dim3((imax + 63) / 64, (jmax + 3) / 4), dim3(256), 0, s, u, v, ub1, imax,
-                       jmax, pitch, inv_dm, inv_dn, out);
+                       jmax, pitch, inv_dm, inv_dn, (int)xper, out);
     return hipGetLastError();
 }
 #if defined(NS_KERNELS_F64)
 template hipError_t launch_vorticity<double>(const double *, const double *, const double *, int, int, int, double,
-                                             double, double *, hipStream_t);
+                                             double, bool, double *, hipStream_t);
 #endif
 #if defined(NS_KERNELS_F32)
 template hipError_t launch_vorticity<float>(const float *, const float *, const double *, int, int, int, double,
-                                            double, double *, hipStream_t);
+                                            double, bool, double *, hipStream_t);
 #endif
 
 // Kinetic energy at x-stations (outt.f90:92-110): for column i,

@@ -9,7 +9,7 @@ module ns_capi
 
   integer(c_int32_t), parameter :: NS_OK = 0
   integer(c_int32_t), parameter :: NS_PREC_FP64 = 0, NS_PREC_FP32_PERT = 1
-  integer(c_int32_t), parameter :: NS_FLAG_INVISCID = 1, NS_FLAG_METRIC_EVERYWHERE = 2
+  integer(c_int32_t), parameter :: NS_FLAG_INVISCID = 1, NS_FLAG_METRIC_EVERYWHERE = 2, NS_FLAG_PERIODIC_X = 4
 
   type, bind(C) :: ns_params
     integer(c_int32_t) :: imax, jmax, nk, D
@@ -136,6 +136,55 @@ module ns_capi
       real(c_double), value :: ct
       integer(c_int32_t), value :: iter
       real(c_double), intent(out) :: s(*)
+    end function
+    integer(c_int32_t) function ns_transport(ctx, T, MI, K1) bind(C, name="ns_transport")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: T(*)
+      real(c_double), intent(out) :: MI(*), K1(*)
+    end function
+    integer(c_int32_t) function ns_deriv(ctx, nk, U, T, K1, dUdm, dUdn, dTdm, dTdn, d2Tdm, d2Tdn, psiDiff, &
+                                         d2psidm, d2psidn) bind(C, name="ns_deriv")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nk
+      real(c_double), intent(in) :: U(*), T(*), K1(*)
+      real(c_double), intent(out) :: dUdm(*), dUdn(*), dTdm(*), dTdn(*), d2Tdm(*), d2Tdn(*)
+      real(c_double), intent(out) :: psiDiff(*), d2psidm(*), d2psidn(*)
+    end function
+    integer(c_int32_t) function ns_kdt(ctx, K1, dTdm, dTdn) bind(C, name="ns_kdt")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: K1(*)
+      real(c_double), intent(inout) :: dTdm(*), dTdn(*)
+    end function
+    integer(c_int32_t) function ns_tau_ij(ctx, nk, MI, dUdm, dUdn, tau, dtaudm, dtaudn) bind(C, name="ns_tau_ij")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nk
+      real(c_double), intent(in) :: MI(*), dUdm(*), dUdn(*)
+      real(c_double), intent(out) :: tau(*), dtaudm(*), dtaudn(*)
+    end function
+    integer(c_int32_t) function ns_eulernonlinear(ctx, nk, dUdt, U, dUdm, dUdn, tau, dtaudm, dtaudn, d2Tdm, &
+                                                  d2Tdn, d2psidm, d2psidn, s_acou) bind(C, name="ns_eulernonlinear")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nk
+      real(c_double), intent(out) :: dUdt(*)
+      real(c_double), intent(in) :: U(*), dUdm(*), dUdn(*), tau(*), dtaudm(*), dtaudn(*)
+      real(c_double), intent(in) :: d2Tdm(*), d2Tdn(*), d2psidm(*), d2psidn(*), s_acou(*)
+    end function
+    integer(c_int32_t) function ns_metrics_x(ctx, nk, dU) bind(C, name="ns_metrics_x")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nk
+      real(c_double), intent(inout) :: dU(*)
+    end function
+    integer(c_int32_t) function ns_metrics_y(ctx, nk, dU) bind(C, name="ns_metrics_y")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nk
+      real(c_double), intent(inout) :: dU(*)
     end function
   end interface
 

@@ -57,6 +57,8 @@ def make_params(case: Case, device: int = 0, precision: int = _lib.NS_PREC_FP64)
         flags |= _lib.NS_FLAG_INVISCID
     if case.metric_everywhere:
         flags |= _lib.NS_FLAG_METRIC_EVERYWHERE
+    if getattr(case, "periodic_x", False):
+        flags |= _lib.NS_FLAG_PERIODIC_X
     p.flags = flags
     p.device = device
     return p
@@ -236,9 +238,19 @@ class Solver:
         _lib.check(self.lib.ns_adj_set_arc(self.ctx, ij.shape[0], ij.ctypes.data_as(C.POINTER(C.c_int32))))
 
     def adj_forward(self, iter0: int, nsteps: int, every: int) -> float:
+        """Runs nsteps from the current state keeping checkpoints; returns J.
+        Each call replaces the checkpoints an earlier one kept (the backward
+        always differentiates the latest forward): `adj_generation` counts
+        the calls."""
         J = C.c_double()
         _lib.check(self.lib.ns_adj_forward(self.ctx, int(iter0), int(nsteps), int(every), C.byref(J)))
+        self.adj_generation = getattr(self, "adj_generation", 0) + 1
         return J.value
+
+    def adj_set_keep(self, keep: bool):
+        """Whether the backward may keep a segment's stage inputs (default)
+        or recomputes them per step; both give the same bits."""
+        _lib.check(self.lib.ns_adj_set_keep(self.ctx, int(bool(keep))))
 
     def adj_backward(self) -> np.ndarray:
         out = np.empty(self.shape)

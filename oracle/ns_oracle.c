@@ -26,6 +26,11 @@
  *   nso_bufferxy              closed forms (sigma=1 identity, sigma=0 reset,
  *                             corners Ub + sx*sy*(U-Ub))
  *   nso_acousticfont          closed form at the source centre
+ *   periodic_x (no reference counterpart): the ring sweep against an
+ *                             independent triangular-solve formulation on the
+ *                             ring and its Fourier-mode response away from the
+ *                             seam; wrapped DRP against the closed-form
+ *                             wavenumber of a periodic mode
  *   nso_rk_euler composed over many steps: parity unpinned against the
  *                             reference (rk.f90, filtering.f90, eqeuler.f90,
  *                             derivs.f90, rhs.f90 do not compile as shipped).
@@ -169,6 +174,16 @@ static void drp_line(double *out, size_t ost, const double *u, size_t st, int n,
 #undef U_
 }
 
+/* periodic line (periodic_x): the interior stencil everywhere, indices wrapped */
+static void drp_line_periodic(double *out, size_t ost, const double *u, size_t st, int n, double h)
+{
+#define UW_(q) u[(size_t)(((q) % n + n) % n) * st]
+    for (int q = 0; q < n; ++q)
+        out[(size_t)q * ost] = (A_INT[0] * (UW_(q + 1) - UW_(q - 1))) / h + (A_INT[1] * (UW_(q + 2) - UW_(q - 2))) / h +
+                               (A_INT[2] * (UW_(q + 3) - UW_(q - 3))) / h;
+#undef UW_
+}
+
 /* metric region tests, diff.f90:315-350 (metricsX) and :276-311 (metricsY) */
 static int in_xstrip(const nso_params *p, int i1) { return i1 <= p->D || i1 >= p->imax - p->D + 1; }
 static int in_ystrip(const nso_params *p, int j1) { return j1 <= p->D || j1 >= p->jmax - p->D + 1; }
@@ -178,6 +193,10 @@ static double metx_factor_count(const nso_params *p, int i1, int j1, int *cnt)
     if (p->metric_everywhere) {
         *cnt = 1;
         return p->meshx[i1 - 1];
+    }
+    if (p->periodic_x) { /* no x layers */
+        *cnt = 0;
+        return 1.0;
     }
     const int D = p->D, imax = p->imax, jmax = p->jmax;
     int c = 0;
@@ -215,7 +234,8 @@ void nso_deropm(const nso_params *p, int nk, double *dU, const double *U)
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < jmax; ++j)
         for (int k = 0; k < nk; ++k)
-            drp_line(dU + IDX(p, nk, k, 0, j), (size_t)nk, U + IDX(p, nk, k, 0, j), (size_t)nk, imax, p->dm);
+            (p->periodic_x ? drp_line_periodic : drp_line)(dU + IDX(p, nk, k, 0, j), (size_t)nk,
+                                                           U + IDX(p, nk, k, 0, j), (size_t)nk, imax, p->dm);
     /* metricsX: du = du*meshx(i) once per satisfied region test */
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < jmax; ++j)
@@ -362,9 +382,11 @@ void nso_rk_steep(const nso_params *p, double alphat, double betat, double *U, c
 {
     const int nk = p->nk, imax = p->imax, jmax = p->jmax;
     const double dt = p->dt;
+    /* periodic_x: every column is interior */
+    const int i0 = p->periodic_x ? 0 : 1, i1 = p->periodic_x ? imax : imax - 1;
 #pragma omp parallel for schedule(static)
     for (int j = 1; j < jmax - 1; ++j)
-        for (int i = 1; i < imax - 1; ++i)
+        for (int i = i0; i < i1; ++i)
             for (int k = 0; k < nk; ++k) {
                 size_t q = IDX(p, nk, k, i, j);
                 w[q] = alphat * w[q] + dt * dudt[q];
@@ -393,6 +415,7 @@ void nso_bufferxy(const nso_params *p, double *U)
                     size_t q = IDX(p, nk, k, i, j);
                     U[q] = Ub[q] + p->sigmay[j] * (U[q] - Ub[q]);
                 }
+            if (p->periodic_x) continue; /* no x layers */
             if (i1 >= 1 && i1 <= D)
                 for (int k = 0; k < nk; ++k) {
                     size_t q = IDX(p, nk, k, i, j);
@@ -434,12 +457,33 @@ static void filter_line(double *u, size_t st, int n, double sigmaf, const double
 #undef U_
 }
 
+/* periodic_x: the in-place sweep of the interior stencil around the ring,
+ * i = 1..n (0-based 0..n-1) in order, every index wrapped: at q the points
+ * q-6..q-1 hold values updated in this sweep except those left of the seam
+ * (q-s < 0), and q+1..q+6 hold unfiltered values except those past the end
+ * (q+s >= n), which are the points 0..5 updated at the start. */
+static void filter_ring(double *u, size_t st, int n, double sigmaf)
+{
+#define UW_(q) u[(size_t)(((q) % n + n) % n) * st]
+    for (int q = 0; q < n; ++q) {
+        double Du = 0.0;
+        for (int s = -6; s <= 6; ++s) Du = Du + DF[s + 6] * UW_(q + s);
+        UW_(q) = UW_(q) - sigmaf * Du;
+    }
+#undef UW_
+}
+
 void nso_filterx(const nso_params *p, double *U)
 {
     const int nk = p->nk, imax = p->imax, jmax = p->jmax;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < jmax; ++j)
-        for (int k = 0; k < nk; ++k) filter_line(U + IDX(p, nk, k, 0, j), (size_t)nk, imax, p->sigmaf, DFB_X);
+        for (int k = 0; k < nk; ++k) {
+            if (p->periodic_x)
+                filter_ring(U + IDX(p, nk, k, 0, j), (size_t)nk, imax, p->sigmaf);
+            else
+                filter_line(U + IDX(p, nk, k, 0, j), (size_t)nk, imax, p->sigmaf, DFB_X);
+        }
 }
 
 void nso_filtery(const nso_params *p, double *U)
@@ -457,8 +501,8 @@ void nso_ccpml(const nso_params *p, double *U)
     const int nk = p->nk, imax = p->imax, jmax = p->jmax;
     const double g = p->gamma;
 #define V(k, i, j) U[IDX(p, nk, k, i, j)]
-    /* inflow, i = 1 */
-    for (int j = 0; j < jmax; ++j) {
+    /* inflow, i = 1 (periodic_x: no inflow or outflow column) */
+    for (int j = 0; j < (p->periodic_x ? 0 : jmax); ++j) {
         V(0, 0, j) = p->rhob[j];
         V(1, 0, j) = 2.0 * V(1, 1, j) - V(1, 2, j);
         V(2, 0, j) = 0.0;
@@ -466,7 +510,7 @@ void nso_ccpml(const nso_params *p, double *U)
         if (nk >= 5) V(4, 0, j) = p->psib[j];
     }
     /* outflow, i = imax */
-    for (int j = 0; j < jmax; ++j) {
+    for (int j = 0; j < (p->periodic_x ? 0 : jmax); ++j) {
         V(0, imax - 1, j) = 2.0 * V(0, imax - 2, j) - V(0, imax - 3, j);
         V(1, imax - 1, j) = 2.0 * V(1, imax - 2, j) - V(1, imax - 3, j);
         V(2, imax - 1, j) = 2.0 * V(2, imax - 2, j) - V(2, imax - 3, j);

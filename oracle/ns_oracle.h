@@ -34,6 +34,15 @@ typedef struct nso_params {
        extension of this build for stretched grids: the reference has no
        such switch) */
     int metric_everywhere;
+    /* 1: periodic in x (an extension of this build for the temporal mixing
+       layer; the reference has no periodic boundary): x derivatives use the
+       interior stencil with wrapped indices, there are no x buffer layers,
+       no x metric layers and no inflow/outflow columns, the LDDRK update
+       covers every column, and the x filter is an in-place sweep around the
+       ring from i = 1 (the points left of the seam still hold their
+       unfiltered values when the sweep starts, the points right of the end
+       their filtered ones when it finishes) */
+    int periodic_x;
 } nso_params;
 
 /* RHS callback used by nso_rk_euler_cb: dudt = F(U, stage) */

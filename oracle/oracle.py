@@ -37,6 +37,7 @@ class NSOParams(C.Structure):
         ("meshx", _DP), ("meshy", _DP), ("sigmax", _DP), ("sigmay", _DP), ("m", _DP),
         ("n", _DP), ("rhob", _DP), ("psib", _DP), ("Ub", _DP),
         ("metric_everywhere", C.c_int),
+        ("periodic_x", C.c_int),
     ]
 
 
@@ -101,6 +102,7 @@ class Oracle:
         (p.meshx, p.meshy, p.sigmax, p.sigmay, p.m, p.n, p.rhob, p.psib, p.Ub) = [
             x.ctypes.data_as(_DP) for x in self._keep]
         p.metric_everywhere = int(bool(getattr(c, "metric_everywhere", False)))
+        p.periodic_x = int(bool(getattr(c, "periodic_x", False)))
         self.p = p
         self.shape = (c.jmax, c.imax, c.nk)
 

@@ -23,13 +23,14 @@ static int fail(const char *what, int imax, int jmax, int D, int rb, int mode, i
     return 1;
 }
 
-// mode 0: fast zone; 1: general region; 2: general region with fast blocks
-static int check(int imax, int jmax, int D, int rb, int mode, int H, long &layouts)
+// mode 0: fast zone; 1: general region; 2: general region with fast blocks;
+// xper: periodic x (every column marched, windows wrap within one period)
+static int check(int imax, int jmax, int D, int rb, int mode, int H, long &layouts, bool xper = false)
 {
     MarchMap m{};
-    const bool ok = mode == 0 ? march_region(true, D, imax, jmax, rb, m)
-                    : mode == 1 ? march_region(false, D, imax, jmax, rb, m)
-                                : march_hybrid(D, imax, jmax, rb, m);
+    const bool ok = mode == 0 ? march_region(true, D, imax, jmax, rb, m, xper)
+                    : mode == 1 ? march_region(false, D, imax, jmax, rb, m, xper)
+                                : march_hybrid(D, imax, jmax, rb, m, xper);
     if (!ok) return 0;
     if ((m.y_b - m.y_a) < rb || H % rb) return 0;
     m.H = H;
@@ -48,14 +49,26 @@ static int check(int imax, int jmax, int D, int rb, int mode, int H, long &layou
             // the stored outputs' window: rows y0-6 .. y1+5 (level-1 rows
             // y0-3 .. y1+2), columns i0-6 .. ie+5 (lanes past ie load clamped
             // columns and store nothing)
-            if (ie <= i0 || y0 - 6 < 0 || y1 + 5 > jmax - 1 || i0 - 6 < 0 || ie + 5 > imax - 1)
+            if (ie <= i0 || y0 - 6 < 0 || y1 + 5 > jmax - 1)
                 return fail("window outside the grid", imax, jmax, D, rb, mode, H);
-            // every derivative the stored outputs depend on is the interior one
-            if (y0 - 3 < 3 || y1 + 2 > jmax - 4 || i0 - 3 < 3 || ie + 2 > imax - 4)
+            if (xper) {
+                // the whole 70-column window (lanes past ie too) wraps into
+                // the grid with one period: wrap_col's range
+                if (i0 - 6 < -imax || i0 + 63 >= 2 * imax || wrap_col(i0 - 6, imax) < 0 ||
+                    wrap_col(i0 + 63, imax) >= imax)
+                    return fail("periodic window beyond one period", imax, jmax, D, rb, mode, H);
+            } else if (i0 - 6 < 0 || ie + 5 > imax - 1) {
+                return fail("window outside the grid", imax, jmax, D, rb, mode, H);
+            }
+            // every derivative the stored outputs depend on is the interior
+            // one (periodic x: always along x)
+            if (y0 - 3 < 3 || y1 + 2 > jmax - 4 || (!xper && (i0 - 3 < 3 || ie + 2 > imax - 4)))
                 return fail("one-sided stencil in a marching block", imax, jmax, D, rb, mode, H);
             const bool fast = mode == 0 || (mode == 2 && march_block_fast(m, i0, y0, y1));
-            if (fast && (i0 - 3 < Dm || ie + 3 > imax - Dm || y0 - 3 < Dm || y1 + 3 > jmax - Dm))
+            if (fast && (y0 - 3 < Dm || y1 + 3 > jmax - Dm ||
+                         (!xper && (i0 - 3 < Dm || ie + 3 > imax - Dm))))
                 return fail("fast block outside the fast zone", imax, jmax, D, rb, mode, H);
+            if (xper && (m.xa != 0 || m.xe != imax)) return fail("periodic columns", imax, jmax, D, rb, mode, H);
             // a hybrid block on the fast code covers whole fast-zone columns
             if (mode == 2 && march_block_fast(m, i0, y0, y1) && ie != i0 + 58)
                 return fail("partial strip on the fast code", imax, jmax, D, rb, mode, H);
@@ -140,7 +153,10 @@ int main()
                 for (int rb : {4, 8})
                     for (int mode = 0; mode < 3; ++mode)
                         for (int H : {rb, 4 * rb, 7 * rb, 16 * rb, 50 * rb, 4096})
-                            if (check(imax, jmax, D, rb, mode, H - H % rb, layouts)) return 1;
+                            for (bool xper : {false, true})
+                                if ((!xper || imax >= 64) &&
+                                    check(imax, jmax, D, rb, mode, H - H % rb, layouts, xper))
+                                    return 1;
     for (int imax : sizes)
         for (int jmax : sizes)
             for (int ty : {8, 16})

@@ -196,6 +196,47 @@ def test_autograd_function():
     s.close()
 
 
+def test_kept_and_recomputed_stage_inputs_give_the_same_gradient():
+    """The backward keeps a segment's stage inputs when they fit, else
+    recomputes them per step; ns_adj_set_keep(0) forces the second path.
+    Both run the same kernels on the same inputs: the same bits."""
+    case, arc, s = _objective_setup(N=64, D=8, radius=20.0)
+    dp = case.U0[..., 3] - case.Ub[..., 3]
+    out = {}
+    for keep in (True, False, True):
+        s.adj_set_keep(keep)
+        J = _J(s, case, dp, 17, every=7)
+        out.setdefault(keep, []).append((J, s.adj_backward()))
+    (Jk, gk), (Jk2, gk2) = out[True]
+    (Jr, gr), = out[False]
+    assert Jk == Jr == Jk2
+    assert np.array_equal(gk, gr) and np.array_equal(gk, gk2)
+    s.close()
+
+
+def test_two_objectives_on_one_solver():
+    """(J1 + J2).backward() with both objectives built on one solver: each
+    backward differentiates its own forward (the solver keeps the latest
+    forward's checkpoints only, so the earlier objective re-runs its own)."""
+    import torch
+    from navierstokes_amd.adjoint import observer_objective
+    case, arc, s = _objective_setup(N=64, D=8, radius=20.0)
+    dp1 = case.U0[..., 3] - case.Ub[..., 3]
+    dp2 = 0.5 * np.roll(dp1, 7, axis=1)
+    refs = []
+    for dp in (dp1, dp2):
+        _J(s, case, dp, 10, every=4)
+        refs.append(s.adj_backward()[..., 3])
+    t1 = torch.tensor(dp1, dtype=torch.float64, device="cuda", requires_grad=True)
+    t2 = torch.tensor(dp2, dtype=torch.float64, device="cuda", requires_grad=True)
+    J1 = observer_objective(t1, s, 10, 4)
+    J2 = observer_objective(t2, s, 10, 4)
+    (J1 + J2).backward()
+    assert np.array_equal(t1.grad.cpu().numpy(), refs[0])
+    assert np.array_equal(t2.grad.cpu().numpy(), refs[1])
+    s.close()
+
+
 def test_adjoint_rejects_viscous_cases():
     from navierstokes_amd._lib import NSError
     case = config.random_case(48, 40, nk=4, D=6, seed=5)

@@ -227,7 +227,7 @@ def test_lddrk_stage_times_integrate_cubics():
 # ---------------------------------------------------------------------------
 # RHS: manufactured solution (primitive-variable Navier-Stokes)
 # ---------------------------------------------------------------------------
-def _spectral_fields(N, L):
+def _spectral_fields(N, L, nk=4):
     x = np.arange(N) * L / N
     X, Y = np.meshgrid(x, x)
     kx = 2 * np.pi / L
@@ -235,7 +235,10 @@ def _spectral_fields(N, L):
     u = 0.3 + 0.05 * np.cos(kx * X + 0.3) * np.sin(2 * kx * Y)
     v = -0.1 + 0.04 * np.sin(2 * kx * X) * np.sin(kx * Y + 0.2)
     p = 1 / 1.4 + 0.03 * np.cos(kx * X - 0.4) * np.cos(kx * Y)
-    return X, Y, np.stack([rho, u, v, p], axis=-1)
+    fields = [rho, u, v, p]
+    if nk >= 5:
+        fields.append(0.5 + 0.2 * np.sin(kx * X + 0.7) * np.cos(2 * kx * Y - 0.1))
+    return X, Y, np.stack(fields, axis=-1)
 
 
 def _dx(f, L, axis):
@@ -246,7 +249,7 @@ def _dx(f, L, axis):
     return np.real(np.fft.ifft(np.fft.fft(f, axis=axis) * k.reshape(shape), axis=axis))
 
 
-def _pde_rhs(Uf, L, Re, Pr, tc, gamma=1.4, Tref=273.15):
+def _pde_rhs(Uf, L, Re, Pr, tc, gamma=1.4, Tref=273.15, Pe=1000.0, alpha_cond=0.5):
     rho, u, v, p = (Uf[..., k] for k in range(4))
     dx = lambda f: _dx(f, L, 1)  # noqa: E731
     dy = lambda f: _dx(f, L, 0)  # noqa: E731
@@ -265,6 +268,14 @@ def _pde_rhs(Uf, L, Re, Pr, tc, gamma=1.4, Tref=273.15):
     out[..., 2] = -(u * vx + v * vy + dy(p) / rho) + (dx(txy) + dy(tyy)) / (Re * rho)
     out[..., 3] = (-(u * dx(p) + v * dy(p) + gamma * p * (ux + vy)) + (gamma - 1) / Re * phi
                    + (dx(kk * dx(T)) + dy(kk * dy(T))) / (Re * Pr))
+    if Uf.shape[-1] >= 5:
+        # passive scalar: derivs.f90:36-43 forms the flux rho*alpha*(psi_x + psi_y)
+        # (the sum of both derivatives) and differentiates it in both
+        # directions; eqeuler.f90:94-95 divides by Pe*rho
+        psi = Uf[..., 4]
+        px, py = dx(psi), dy(psi)
+        F = rho * alpha_cond * (px + py)
+        out[..., 4] = -(u * px + v * py - (dx(F) + dy(F)) / (Pe * rho))
     return out
 
 
@@ -280,6 +291,62 @@ def test_rhs_converges_to_navier_stokes():
         inner = slice(N // 4, 3 * N // 4)
         errs.append(max(rel_max(got[inner, inner, k], exact[inner, inner, k]) for k in range(4)))
     assert errs[1] < 2e-4 and errs[0] / errs[1] > 8, errs
+
+
+def _scalar_case(N, h):
+    c = config.random_case(N, N, nk=5, D=0, seed=0, mesh_random=False)
+    c.dm = c.dn = h
+    c.Re, c.Pr = 5.0, 0.7
+    # Pe of order one so that the diffusion term is as large as the
+    # advection term (the reference's Pe = 1000 would hide it)
+    c.Pe, c.alpha_cond = 0.8, 0.5
+    return c
+
+
+def test_scalar_rhs_converges_to_advection_diffusion():
+    """The nk=5 passive-scalar equation (derivs.f90:36-43, eqeuler.f90:94-95)
+    at the PDE level: psi_t = -(u psi_x + v psi_y - (d/dx + d/dy)[rho alpha
+    (psi_x + psi_y)] / (Pe rho)); the oracle's fourth-order discretisation
+    converges to it on a periodic manufactured field."""
+    L = 10.0
+    errs = []
+    for N in (48, 96):
+        X, Y, Uf = _spectral_fields(N, L, nk=5)
+        c = _scalar_case(N, L / N)
+        exact = _pde_rhs(Uf, L, c.Re, c.Pr, c.transport(), Pe=c.Pe, alpha_cond=c.alpha_cond)
+        got = Oracle(c).rhs(Uf)
+        inner = slice(N // 4, 3 * N // 4)
+        errs.append(rel_max(got[inner, inner, 4], exact[inner, inner, 4]))
+        # components 1-4 do not see psi
+        four = Oracle(c.with_nk(4)).rhs(Uf[..., :4])
+        assert np.array_equal(got[..., :4], four)
+    assert errs[1] < 2e-4 and errs[0] / errs[1] > 8, errs
+
+
+def test_scalar_pin_rejects_plausible_misreadings():
+    """The scalar pin separates the reference's form from nearby misreadings:
+    each of them differs from the oracle by far more than its discretisation
+    error."""
+    L, N = 10.0, 96
+    X, Y, Uf = _spectral_fields(N, L, nk=5)
+    c = _scalar_case(N, L / N)
+    got = Oracle(c).rhs(Uf)[..., 4]
+    inner = slice(N // 4, 3 * N // 4)
+    rho, u, v, psi = Uf[..., 0], Uf[..., 1], Uf[..., 2], Uf[..., 4]
+    dx = lambda f: _dx(f, L, 1)  # noqa: E731
+    dy = lambda f: _dx(f, L, 0)  # noqa: E731
+    px, py = dx(psi), dy(psi)
+    a, Pe = c.alpha_cond, c.Pe
+    variants = {
+        # Fickian flux: each direction differentiated on its own
+        "fick": -(u * px + v * py - (dx(rho * a * px) + dy(rho * a * py)) / (Pe * rho)),
+        # no 1/rho on the diffusion term
+        "no_inv_rho": -(u * px + v * py - (dx(rho * a * (px + py)) + dy(rho * a * (px + py))) / Pe),
+        # diffusion sign flipped
+        "sign": -(u * px + v * py + (dx(rho * a * (px + py)) + dy(rho * a * (px + py))) / (Pe * rho)),
+    }
+    for name, f in variants.items():
+        assert rel_max(got[inner, inner], f[inner, inner]) > 1e-2, name
 
 
 def test_metric_everywhere_closed_form():
@@ -376,3 +443,120 @@ def test_openmp_build_is_bitwise_serial():
     a = Oracle(c).run(c.U0, 0, 2)
     b = Oracle(c, parallel=True).run(c.U0, 0, 2)
     assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+# periodic x (no reference counterpart: the build's extension for the
+# temporal mixing layer)
+# ---------------------------------------------------------------------------
+def _periodic_case(imax=96, jmax=40, nk=4, seed=5, D=6):
+    c = config.random_case(imax, jmax, nk=nk, D=D, seed=seed, mesh_random=True)
+    c.periodic_x = True
+    return c
+
+
+def filter_ring_matrix(u, sigma):
+    """The ring sweep as one linear solve: with D the circulant 13-point
+    filter and L its part on points the sweep has already updated (column
+    (q+s) mod n < q), the sweep is (I + sigma L) u' = (I - sigma (D - L)) u,
+    L strictly lower triangular."""
+    n = u.shape[0]
+    Dm = np.zeros((n, n))
+    for q in range(n):
+        for s in range(-6, 7):
+            Dm[q, (q + s) % n] += DF[s + 6]
+    L = np.tril(Dm, -1)
+    rhs = u - sigma * ((Dm - L) @ u)
+    return solve_triangular(np.eye(n) + sigma * L, rhs, lower=True)
+
+
+def test_periodic_filter_against_triangular_solve_and_fourier_mode():
+    c = _periodic_case(96, 20)
+    o = Oracle(c)
+    rng = np.random.default_rng(8)
+    U = rng.standard_normal((20, 96, 4))
+    got = o.filterx(U)
+    for j in (0, 7, 19):
+        ref = filter_ring_matrix(U[j, :, 2], c.sigmaf)
+        assert rel_max(got[j, :, 2], ref) < 1e-14
+    # the y filter is untouched by the x periodicity
+    c2 = _periodic_case(96, 20)
+    c2.periodic_x = False
+    assert np.array_equal(o.filtery(U), Oracle(c2).filtery(U))
+    # a Fourier mode away from the seam: u' = G u with the in-place sweep's
+    # amplification G = (1 - sigma (d0 + A+)) / (1 + sigma A-), A+- the
+    # taps ahead of / behind the point
+    n = 200
+    c3 = _periodic_case(n, 16)
+    s = np.arange(-6, 7)
+    for m in (7, 31, 70):
+        k = 2 * np.pi * m / n
+        x = np.arange(n)
+        Uk = np.zeros((16, n, 4))
+        Uk[..., 1] = np.cos(k * x + 0.3)[None, :]
+        out = Oracle(c3).filterx(Uk)[5, :, 1]
+        ap = np.sum(DF[s > 0] * np.exp(1j * k * s[s > 0]))
+        am = np.sum(DF[s < 0] * np.exp(1j * k * s[s < 0]))
+        G = (1 - c3.sigmaf * (DF[6] + ap)) / (1 + c3.sigmaf * am)
+        exact = np.real(G * np.exp(1j * (k * x + 0.3)))
+        inner = slice(60, n - 7)     # the seam's transient has decayed (0.33^60)
+        assert np.abs(out[inner] - exact[inner]).max() < 1e-14, m
+        assert np.abs(out[:6] - exact[:6]).max() > 1e-11    # the seam is visible
+
+
+def test_periodic_derivative_is_the_wrapped_drp():
+    n = 64
+    c = _periodic_case(n, 12)
+    c.meshx = np.ones(n)
+    c.dm = 0.5
+    o = Oracle(c)
+    a = np.array([0.770882380518, -0.166705904415, 0.0208431427703])
+    for m in (1, 5, 13):
+        kdx = 2 * np.pi * m / n
+        x = np.arange(n)
+        f = np.broadcast_to(np.sin(kdx * x), (12, n)).copy()
+        kstar = 2.0 * sum(a[t] * np.sin((t + 1) * kdx) for t in range(3)) / c.dm
+        assert np.allclose(o.deropm(f), kstar * np.cos(kdx * x)[None, :], rtol=0, atol=1e-13)
+    # no x metric layers: meshx is ignored unless applied everywhere
+    c.meshx = 1.0 + np.random.default_rng(2).random(n)
+    assert np.array_equal(Oracle(c).deropm(f), o.deropm(f))
+
+
+def test_periodic_pointwise_operators():
+    c = _periodic_case(50, 36, nk=5, D=7)
+    c.sigmax = np.random.default_rng(3).random(50)
+    o = Oracle(c)
+    U = c.U0
+    # no inflow/outflow columns; rows j = 1, jmax on every column (cc.f90:38-59)
+    cc = o.ccpml(U)
+    assert np.array_equal(cc[1:-1], U[1:-1])
+    assert np.array_equal(cc[0, :, :3], 2.0 * U[1, :, :3] - U[2, :, :3])
+    assert np.all(cc[0, :, 3] == 1.0 / c.gamma) and np.all(cc[-1, :, 4] == 1.0)
+    # no x buffer layers: the y layers alone
+    b = o.bufferxy(U)
+    D = c.D
+    assert np.array_equal(b[D:-D], U[D:-D])
+    # the LDDRK update reaches every column
+    rng = np.random.default_rng(4)
+    dudt, w = rng.standard_normal(U.shape), rng.standard_normal(U.shape)
+    Un, _ = o.rk_steep(-0.7, 0.8, U, dudt, w)
+    assert not np.array_equal(Un[1:-1, 0], U[1:-1, 0]) and not np.array_equal(Un[1:-1, -1], U[1:-1, -1])
+    assert np.array_equal(Un[0], U[0]) and np.array_equal(Un[-1], U[-1])
+
+
+def test_periodic_step_commutes_with_x_shifts():
+    """Without the filter's seam (sigma_f = 0) every x operation of the
+    periodic step is a wrapped stencil, the same arithmetic at every column:
+    the step commutes with rolling the state in x, bitwise."""
+    c = _periodic_case(60, 48, nk=5, D=8)
+    c.sigmaf = 0.0
+    c.meshx = np.ones(60)
+    o = Oracle(c)
+    a = o.rk_euler(c.U0, 0)
+    for r in (1, 17):
+        cr = _periodic_case(60, 48, nk=5, D=8)
+        cr.sigmaf = 0.0
+        cr.meshx = np.ones(60)
+        cr.U0 = np.roll(c.U0, r, axis=1)
+        b = Oracle(cr).rk_euler(cr.U0, 0)
+        assert np.array_equal(np.roll(a, r, axis=1), b)
