@@ -107,6 +107,16 @@ int32_t ns_put_state(ns_ctx *ctx, const double *U);  /* host AoS -> device */
 int32_t ns_get_state(ns_ctx *ctx, double *U);        /* device -> host AoS */
 /* gather npts points (1-based (i,j) pairs) x nk values: euler.f90:156-161 probes */
 int32_t ns_probe(ns_ctx *ctx, int32_t npts, const int32_t *ij, double *out);
+/* ns_probe without any device allocation (no hipMalloc, hipFree or
+   allocating library call): for callers that cannot allocate, such as
+   harnesses that forbid it or per-step probe loops that must not touch the
+   allocator.  No counterpart in the reference.  The point list and the
+   gathered values are staged in a fixed device array of NS_PROBE_NOALLOC_MAX
+   points, one per device, shared by every call on that device: calls that
+   overlap on one device race.  npts above NS_PROBE_NOALLOC_MAX fails with
+   NS_ERR_PARAM.  Same arguments and results as ns_probe. */
+#define NS_PROBE_NOALLOC_MAX 4096
+int32_t ns_probe_noalloc(ns_ctx *ctx, int32_t npts, const int32_t *ij, double *out);
 /* NS_ERR_NONFINITE if any state value is NaN/Inf */
 int32_t ns_check_finite(ns_ctx *ctx);
 /* outt.f90:49-55: the vorticity column of the snapshot files,

@@ -25,6 +25,8 @@ NS_FLAG_INVISCID = 1
 NS_FLAG_METRIC_EVERYWHERE = 2
 NS_FLAG_PERIODIC_X = 4
 
+NS_PROBE_NOALLOC_MAX = 4096
+
 
 class NSParams(C.Structure):
     _fields_ = [
@@ -67,6 +69,7 @@ SIGNATURES = {
     "ns_put_state": (C.c_int32, [_V, _DP]),
     "ns_get_state": (C.c_int32, [_V, _DP]),
     "ns_probe": (C.c_int32, [_V, C.c_int32, _IP, _DP]),
+    "ns_probe_noalloc": (C.c_int32, [_V, C.c_int32, _IP, _DP]),
     "ns_check_finite": (C.c_int32, [_V]),
     "ns_row_average": (C.c_int32, [_V, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "ns_vorticity": (C.c_int32, [_V, _DP]),

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -793,6 +794,53 @@ int32_t ns_default_params(ns_params *p)
     return NS_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Staging of ns_probe_noalloc without device allocation: the point list and
+// the gathered values live in one file-scope device array.  Each device holds
+// its own copy of it at its own address; the address is looked up once per
+// device (ns_create, with the device current) and kept in scratch_dev.  Host
+// code passes that address, never the array's name, to copies and kernels.
+// Every call on a device shares that device's copy: calls that overlap on one
+// device race.  Nothing in it needs clearing: a call writes the point list
+// and every gathered value before it reads them.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t SCRATCH_IJ_BYTES = sizeof(int32_t) * 2 * NS_PROBE_NOALLOC_MAX;
+constexpr size_t SCRATCH_OUT_OFFSET = (SCRATCH_IJ_BYTES + 255) / 256 * 256;
+constexpr size_t SCRATCH_BYTES = SCRATCH_OUT_OFFSET + sizeof(double) * NKMAX * NS_PROBE_NOALLOC_MAX;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+std::mutex scratch_mu;
+void *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// the device's address of `scratch` (dev current); once per device
+int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return fail(NS_ERR_PARAM, "device %d beyond the %d the library supports", dev, SCRATCH_MAX_DEVICES);
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (scratch_dev[dev]) return NS_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(NS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    scratch_dev[dev] = ptr;
+    return NS_OK;
+}
+
+void *scratch_ptr(int dev)
+{
+    std::lock_guard<std::mutex> g(scratch_mu);
+    return (dev >= 0 && dev < SCRATCH_MAX_DEVICES) ? scratch_dev[dev] : nullptr;
+}
+}  // namespace
+
+extern "C" {
+
 int32_t ns_create(const ns_params *p, ns_ctx **out)
 {
     if (!p || !out) return fail(NS_ERR_PARAM, "null argument");
@@ -828,6 +876,8 @@ int32_t ns_create(const ns_params *p, ns_ctx **out)
     };
     hipError_t e = hipSetDevice(P.device);
     if (e != hipSuccess) return bail(fail(NS_ERR_HIP, "hipSetDevice(%d): %s", P.device, hipGetErrorString(e)));
+    int src;
+    if ((src = scratch_setup(P.device))) return bail(src);
     int lds = 0;
     hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, P.device);
     if (lds < 150 * 1024)
@@ -1011,6 +1061,36 @@ int32_t ns_probe(ns_ctx *c, int32_t npts, const int32_t *ij, double *out)
     HIPCHK(hipStreamSynchronize(c->stream));
     hipFree(dij);
     hipFree(dout);
+    if (c->pert)
+        for (int t = 0; t < npts; ++t)
+            for (int k = 0; k < nk; ++k) out[t * nk + k] += c->ubh[(size_t)k * c->P.jmax + (ij[2 * t + 1] - 1)];
+    return NS_OK;
+}
+
+int32_t ns_probe_noalloc(ns_ctx *c, int32_t npts, const int32_t *ij, double *out)
+{
+    if (!c || npts < 0 || (npts > 0 && (!ij || !out))) return fail(NS_ERR_PARAM, "bad argument");
+    if (npts > NS_PROBE_NOALLOC_MAX)
+        return fail(NS_ERR_PARAM, "%d probe points exceed NS_PROBE_NOALLOC_MAX (%d)", npts, NS_PROBE_NOALLOC_MAX);
+    if (npts == 0) return NS_OK;
+    for (int t = 0; t < npts; ++t)
+        if (ij[2 * t] < 1 || ij[2 * t] > c->P.imax || ij[2 * t + 1] < 1 || ij[2 * t + 1] > c->P.jmax)
+            return fail(NS_ERR_PARAM, "probe %d at (%d,%d) lies outside the grid", t, ij[2 * t], ij[2 * t + 1]);
+    HIPCHK(hipSetDevice(c->P.device));
+    unsigned char *base = static_cast<unsigned char *>(scratch_ptr(c->P.device));
+    if (!base) return fail(NS_ERR_STATE, "device %d has no probe staging (no context was created on it)", c->P.device);
+    int *dij = reinterpret_cast<int *>(base);
+    double *dout = reinterpret_cast<double *>(base + SCRATCH_OUT_OFFSET);
+    const int nk = c->P.nk;
+    int rc = set_ptr_table(c, 0, c->S[0], nk, c->esz);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(dij, ij, sizeof(int) * 2 * npts, hipMemcpyHostToDevice, c->stream));
+    if (c->pert)
+        HIPCHK(launch_probe<float>((const float *const *)c->dptrs, nk, c->pitch, npts, dij, dout, c->stream));
+    else
+        HIPCHK(launch_probe<double>((const double *const *)c->dptrs, nk, c->pitch, npts, dij, dout, c->stream));
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * npts * nk, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (c->pert)
         for (int t = 0; t < npts; ++t)
             for (int k = 0; k < nk; ++k) out[t * nk + k] += c->ubh[(size_t)k * c->P.jmax + (ij[2 * t + 1] - 1)];

@@ -117,12 +117,14 @@ class Solver:
         _lib.check(self.lib.ns_get_state(self.ctx, _lib.dptr(U)))
         return U
 
-    def probe(self, points) -> np.ndarray:
-        """Values at 1-based (i, j) points: shape (npts, nk) (euler.f90:156-161)."""
+    def probe(self, points, noalloc: bool = False) -> np.ndarray:
+        """Values at 1-based (i, j) points: shape (npts, nk) (euler.f90:156-161).
+        noalloc: stage through the library's fixed device array instead of
+        allocating (at most NS_PROBE_NOALLOC_MAX points)."""
         ij = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 2))
         out = np.empty((ij.shape[0], self.case.nk))
-        _lib.check(self.lib.ns_probe(self.ctx, ij.shape[0], ij.ctypes.data_as(C.POINTER(C.c_int32)),
-                                     _lib.dptr(out)))
+        fn = self.lib.ns_probe_noalloc if noalloc else self.lib.ns_probe
+        _lib.check(fn(self.ctx, ij.shape[0], ij.ctypes.data_as(C.POINTER(C.c_int32)), _lib.dptr(out)))
         return out
 
     def vorticity(self) -> np.ndarray:
