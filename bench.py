@@ -417,6 +417,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="ns/pulse: skip the FP32-perturbation leg")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--variant", type=int, default=-1, help="stage variant 0..2 (default: the autotuned one)")
+    ap.add_argument("--filter-mode", type=int, default=-1,
+                    help="0: x and y filter sweeps as two kernels, 1: one fused sweep (default: autotuned)")
     ap.add_argument("--precision", choices=["fp64", "fp32pert"], default=None,
                     help="default fp64 (ns, pulse) / fp32pert (batch)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -471,7 +473,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "grid": [case.imax, case.jmax], "nk": nk, "precision": args.precision,
                    "viscous": not case.inviscid, "parallelism": "single-gpu", "stage_variant": variant,
-                   "stage_variant_ms": [round(x, 4) for x in tune_ms]},
+                   "stage_variant_ms": [round(x, 4) for x in tune_ms], "filter_mode": r["filter_mode"],
+                   "filter_tune_ms": r["filter_tune_ms"]},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(args.workload, case.imax, args.precision, variant),
@@ -497,6 +500,8 @@ def main():
             "pprime_rel_l2_vs_fp64": float(np.linalg.norm(r32["pprime"] - r["pprime"]) / np.linalg.norm(r["pprime"])),
             "steps_compared": r["nsteps"],
             "stage_variant": r32["variant"],
+            "filter_mode": r32["filter_mode"],
+            "filter_tune_ms": r32["filter_tune_ms"],
             "kernel_ms": {k: round(v, 4) for k, v in kt32.items()},
             "interior_roofline_frac": round(ach32 / HBM_PEAK_GBS, 4),
         }
@@ -518,6 +523,9 @@ def run_form(case, desc, prec, args):
     if args.variant >= 0:
         s.set_variant(args.variant)
         variant = args.variant
+    if args.filter_mode >= 0:
+        s.set_filter_mode(args.filter_mode)
+    fmode, fms = s.filter_mode()
     s.run(0, args.warmup)
     s.synchronize()
     print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
@@ -536,7 +544,8 @@ def run_form(case, desc, prec, args):
     s.close()
     return {"ms_step": t / args.steps * 1e3, "value": case.npoints * args.steps / t / 1e6, "kt": kt,
             "npts_int": npts_int, "variant": variant, "tune_ms": tune_ms, "pprime": pprime,
-            "nsteps": args.warmup + args.steps + args.timing_steps}
+            "nsteps": args.warmup + args.steps + args.timing_steps, "filter_mode": fmode,
+            "filter_tune_ms": [round(float(x), 4) for x in fms]}
 
 
 if __name__ == "__main__":

@@ -275,6 +275,19 @@ int32_t ns_autotune(ns_ctx *ctx, int32_t *variant, double *ms);
    No reference counterpart. */
 int32_t ns_set_variant(ns_ctx *ctx, int32_t variant);
 
+/* The stage's two filter sweeps (filtering.f90:9-190): mode 0 runs the x and
+   the y sweep as two kernels, mode 1 as one fused sweep that reads and writes
+   the state once (each 64-column strip keeps its x-filtered rows in LDS for
+   the y recurrence).  Both start each chunk's recurrences W points early
+   from unfiltered values (exact to 1e-19 relative), so they agree to
+   rounding; lines shorter than 272 points are swept in the reference's
+   order by both (bitwise).  ns_autotune picks the faster mode unless this
+   call pinned one; ns_get_filter_mode reports the mode and (ms[2], may be
+   NULL) the tuned times of both (-1: not tuned).  No reference
+   counterpart. */
+int32_t ns_set_filter_mode(ns_ctx *ctx, int32_t mode);
+int32_t ns_get_filter_mode(ns_ctx *ctx, int32_t *mode, double *ms);
+
 /* measurement ----------------------------------------------------------------- */
 /* average device time (ms) of each kernel class over the last ns_rk_steps
    call with timing enabled: names "rhs" (the whole stage launch: interior

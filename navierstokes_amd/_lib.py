@@ -94,6 +94,8 @@ SIGNATURES = {
     "ns_metrics_y": (C.c_int32, [_V, C.c_int32, _DP]),
     "ns_autotune": (C.c_int32, [_V, _IP, _DP]),
     "ns_set_variant": (C.c_int32, [_V, C.c_int32]),
+    "ns_set_filter_mode": (C.c_int32, [_V, C.c_int32]),
+    "ns_get_filter_mode": (C.c_int32, [_V, _IP, _DP]),
     "ns_stage_interior": (C.c_int32, [_V, C.POINTER(C.c_int64)]),
     "ns_enable_timing": (C.c_int32, [_V, C.c_int32]),
     "ns_kernel_time": (C.c_int32, [_V, C.c_char_p, _DP, _IP]),

@@ -30,6 +30,8 @@ hipError_t prepare_step(const StageArgs<double> &A, int nk, bool visc, int varia
 hipError_t prepare_step(const StageArgs<float> &A, int nk, bool visc, int variant, bool batch);
 hipError_t launch_filter(const FilterArgs<double> &F, bool xdir, hipStream_t s);
 hipError_t launch_filter(const FilterArgs<float> &F, bool xdir, hipStream_t s);
+hipError_t launch_filter_xy(const FilterXYArgs<double> &F, hipStream_t s);
+hipError_t launch_filter_xy(const FilterXYArgs<float> &F, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<double> &E, hipStream_t s);
 hipError_t launch_edge_i(const EdgeArgs<float> &E, hipStream_t s);
 hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
@@ -157,6 +159,10 @@ struct ns_ctx {
     bool have_profiles = false, have_base = false, have_state = false;
     int filt_L = 256, filt_W = 40;
     double filt_ms = -1.0;  // autotuned time of both filter sweeps at filt_L (-1: not tuned)
+    int filt_mode = 0;      // 0: x and y sweeps as two kernels; 1: one fused sweep (filter_xy_kernel)
+    double filt_xy_ms = -1.0;  // autotuned time of the fused sweep (-1: not tuned)
+    int filt_Ly = 512;      // row chunk length of the fused sweep
+    bool filt_pinned = false;  // ns_set_filter_mode chose the mode: the autotuner keeps it
     hipGraphExec_t graph = nullptr;
     int stage_variant = -1;       // block-size variant of the stage kernel (-1: not tuned)
     double tune_ms[3] = {0, 0, 0};
@@ -350,6 +356,43 @@ FilterArgs<S> filter_args(ns_ctx *c, bool xdir, S *in, S *out, bool edge_bc, boo
 }
 
 template <typename S>
+FilterXYArgs<S> filter_xy_args(ns_ctx *c, S *in, S *out, bool pert)
+{
+    const ns_params &P = c->P;
+    FilterXYArgs<S> F{};
+    const PertRows pr = pert ? pert_rows(c) : PertRows{};
+    for (int k = 0; k < NKMAX; ++k) {
+        F.in[k] = k < P.nk ? in + (size_t)k * c->plane : nullptr;
+        F.out[k] = k < P.nk ? out + (size_t)k * c->plane : nullptr;
+        F.corr[k] = (pert && k < P.nk) ? pr.ycorr[k] : nullptr;
+    }
+    F.re = pert ? c->re : fp64_row_edge(P);
+    F.imax = P.imax;
+    F.jmax = P.jmax;
+    F.pitch = c->pitch;
+    // lines shorter than 272 points are swept from their start in one
+    // piece, as the two-kernel sweeps do (the reference's order, bitwise)
+    F.Ly = P.jmax < 272 ? P.jmax : c->filt_Ly;
+    F.x_exact = P.imax < 272;
+    F.W = c->filt_W;
+    F.xper = (P.flags & NS_FLAG_PERIODIC_X) ? 1 : 0;
+    F.sigma = P.sigmaf;
+    F.nk = P.nk;
+    return F;
+}
+
+// the stage's filters: S1 -> (x) S2 -> (y + ccpml rows) out, or S1 -> out
+// in one fused sweep
+template <typename S>
+hipError_t launch_filters(ns_ctx *c, S *S1, S *S2, S *out, bool pert, hipStream_t s)
+{
+    if (c->filt_mode == 1) return launch_filter_xy(filter_xy_args<S>(c, S1, out, pert), s);
+    hipError_t e = launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, s);
+    if (e == hipSuccess) e = launch_filter(filter_args<S>(c, false, S2, out, true, pert), false, s);
+    return e;
+}
+
+template <typename S>
 EdgeArgs<S> edge_args(ns_ctx *c, S *U, bool pert)
 {
     EdgeArgs<S> E{};
@@ -408,9 +451,15 @@ int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed, S *in = nullptr, S
     HIPCHK(hipEventRecord(c->join, c->side));
     HIPCHK(hipStreamWaitEvent(c->stream, c->join, 0));
     tick(1);
-    HIPCHK(launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, c->stream));
-    tick(2);
-    HIPCHK(launch_filter(filter_args<S>(c, false, S2, S0out, true, pert), false, c->stream));
+    if (c->filt_mode == 1) {
+        // one fused sweep: its time is reported under "filter_x", "filter_y" gets 0
+        HIPCHK(launch_filter_xy(filter_xy_args<S>(c, S1, S0out, pert), c->stream));
+        tick(2);
+    } else {
+        HIPCHK(launch_filter(filter_args<S>(c, true, S1, S2, false, pert), true, c->stream));
+        tick(2);
+        HIPCHK(launch_filter(filter_args<S>(c, false, S2, S0out, true, pert), false, c->stream));
+    }
     tick(3);
     if (!(P.flags & NS_FLAG_PERIODIC_X)) HIPCHK(launch_edge_i(edge_args<S>(c, S0out, pert), c->stream));
     tick(4);
@@ -539,6 +588,39 @@ int autotune_stage_t(ns_ctx *c)
     }
     c->filt_L = bestL;
     if (ntune) c->filt_ms = bestL_ms;
+    // the fused sweep (filter_xy_kernel) against the two sweeps, with its
+    // row chunk length; the fused one must be more than 3 % faster to be
+    // chosen (a caller's ns_set_filter_mode stands)
+    if (ntune && !c->filt_pinned) {
+        const int Lys[3] = {256, 512, 1024};
+        int bestLy = c->filt_Ly;
+        double bestxy = 1e300;
+        const int mode0 = c->filt_mode;
+        c->filt_mode = 1;
+        for (int t = 0; t < 3; ++t) {
+            if (Lys[t] >= P.jmax && t > 0) break;
+            c->filt_Ly = Lys[t];
+            auto sweep = [&]() -> hipError_t { return launch_filters<S>(c, S1, S2, S2, c->pert, c->stream); };
+            hipError_t e = sweep();
+            if (e != hipSuccess) {
+                c->filt_mode = mode0;
+                return fail(NS_ERR_HIP, "fused filter: %s", hipGetErrorString(e));
+            }
+            HIPCHK(hipEventRecord(c->ev[0], c->stream));
+            for (int r = 0; r < reps; ++r) HIPCHK(sweep());
+            HIPCHK(hipEventRecord(c->ev[1], c->stream));
+            HIPCHK(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            if (ms / reps < bestxy) {
+                bestxy = ms / reps;
+                bestLy = Lys[t];
+            }
+        }
+        c->filt_Ly = bestLy;
+        c->filt_mode = bestxy < 0.97 * bestL_ms ? 1 : 0;
+        c->filt_xy_ms = bestxy;
+    }
     return NS_OK;
 }
 
@@ -1579,6 +1661,31 @@ int32_t ns_set_variant(ns_ctx *c, int32_t variant)
     c->stage_variant = variant;
     // set up its kernels now, outside any capture the caller may start
     return c->pert ? prepare_capture_t<float>(c) : prepare_capture_t<double>(c);
+}
+
+int32_t ns_set_filter_mode(ns_ctx *c, int32_t mode)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (mode < 0 || mode > 1) return fail(NS_ERR_PARAM, "filter mode must be 0 or 1");
+    if (c->graph) {
+        HIPCHK(hipGraphExecDestroy(c->graph));
+        c->graph = nullptr;
+    }
+    c->filt_mode = mode;
+    c->filt_pinned = true;
+    return c->pert ? prepare_capture_t<float>(c) : prepare_capture_t<double>(c);
+}
+
+int32_t ns_get_filter_mode(ns_ctx *c, int32_t *mode, double *ms)
+{
+    if (!c || !mode) return fail(NS_ERR_PARAM, "null argument");
+    *mode = c->filt_mode;
+    if (ms) {
+        ms[0] = c->filt_ms;
+        ms[1] = c->filt_xy_ms;
+    }
+    return NS_OK;
 }
 
 int32_t ns_enable_timing(ns_ctx *c, int32_t on)

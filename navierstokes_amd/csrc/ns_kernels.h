@@ -96,6 +96,23 @@ struct EulerPwArgs {
     double *out[NKMAX];
 };
 
+// Both filters in one sweep (filter_xy_kernel): x filter from `in`, y filter
+// (with the ccpml rows) into `out`.
+template <typename S>
+struct FilterXYArgs {
+    const S *in[NKMAX];
+    S *out[NKMAX];
+    const double *corr[NKMAX];  // perturbation form: Ly(Ub) - Ub per row (null in FP64)
+    int imax, jmax, pitch;
+    int Ly;       // row chunk length
+    int W;        // look-back of both recurrences
+    int x_exact;  // every strip starts its x recurrence at the left end (small grids: the reference order)
+    int xper;     // periodic x: the x recurrence runs around the ring
+    double sigma;
+    int nk;
+    RowEdge re;
+};
+
 template <typename S>
 struct EdgeArgs {
     S *U[NKMAX];

@@ -1188,6 +1188,7 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
 #pragma unroll
             for (int u = 0; u < 13; ++u) nn[u] = ld(qn + 7 + u);
         } else {
+            io.burst(qn + 7);  // the IO may refill its source before a burst of 13 loads
 #pragma unroll
             for (int u = 0; u < 13; ++u) nb[u] = ld(qn + 7 + u);
         }
@@ -1346,6 +1347,7 @@ struct XIO {
         tout[lane][e - oc] = v;
     }
     __device__ __forceinline__ void finish() { flush(); }
+    __device__ __forceinline__ void burst(int) {}
     __device__ __forceinline__ R load_head(int p)  // this lane's row, position p (unstaged)
     {
         const int gj = row0 + lane < m ? row0 + lane : m - 1;
@@ -1416,6 +1418,7 @@ struct YIO {
         if (active && e >= a && e < b) out[(size_t)e * pitch + col] = corr ? R(double(v) + corr[e]) : v;
     }
     __device__ __forceinline__ void finish() {}
+    __device__ __forceinline__ void burst(int) {}
     __device__ __forceinline__ R load_head(int p) { return in[(size_t)p * pitch + colc]; }
 };
 
@@ -1482,6 +1485,180 @@ __global__ __launch_bounds__(64) void filter_y_kernel_batch(const FilterArgs<R> 
     if ((int)blockIdx.y >= F.nk) return;
     filter_y_tile<R>(F, t.y, t.z, blockIdx.y);
 }
+
+// ---------------------------------------------------------------------------
+// Both filters in one sweep (lddfilterx then lddfiltery, filtering.f90:9-190)
+// for the plane of one variable: one wave owns the 64 columns [c0, c0+64) of
+// a chunk of rows.  It walks its chunk's rows in groups of XY_G = 52; for
+// each group the x phase filters the group's rows along x (lanes = rows,
+// loads staged through LDS for coalescing, the x recurrence started W points
+// left of c0 from unfiltered values, or exactly from the left end) and
+// leaves the strip's x-filtered values in an LDS tile; the y phase (lanes =
+// columns) continues each column's y recurrence through the tile and writes
+// the filtered rows, plus the ccpml rows at the ends, once.  The y phase
+// loads in bursts of 13 positions, and the groups are aligned so that a
+// burst that starts past the tile is exactly the first of the next group:
+// the tile is refilled there and nowhere else.  The state is read and
+// written once per stage instead of twice; the price is the x recurrence's
+// W + 12 extra columns per strip (from L2 for neighbouring strips) and, per
+// row chunk, the y look-back rows.
+// ---------------------------------------------------------------------------
+constexpr int XY_G = 52;  // rows per group: 4 bursts of 13
+
+// x phase IO: lanes = rows row0 + lane; emits the strip's columns into the tile
+template <typename R>
+struct XTIO : XIO<R> {
+    R (*tt)[65];  // XY_G rows x 64 columns (+1 pad)
+    int c0;
+    __device__ __forceinline__ void emit(int e, R v)
+    {
+        if (e >= c0 && e < c0 + 64 && this->lane < XY_G) tt[this->lane][e - c0] = v;
+    }
+    __device__ __forceinline__ void finish() {}
+};
+
+// y phase IO: lane = column c0 + lane; positions are rows, read from the tile
+// (groups of XY_G rows start at the sweep's first load; the sweep's bursts of
+// 13 start there too, so a group boundary is always a burst's start)
+template <typename R, bool RING>
+struct YTIO {
+    const FilterXYArgs<R> *F;
+    int k, c0, lane, col, a, b;
+    bool active;
+    R (*tt)[65];
+    R (*tin)[XIO<R>::T + 1];
+    int g0, gend;  // the tile holds rows [g0, gend)
+    R *out;
+    const double *corr;
+    __device__ __forceinline__ void fill(int g)  // x-filter rows [g, g + XY_G) of the strip into the tile
+    {
+        XTIO<R> x;
+        x.in = F->in[k];
+        x.out = nullptr;
+        x.pitch = F->pitch;
+        x.n = F->imax;
+        x.m = F->jmax;
+        x.row0 = g;
+        x.lane = lane;
+        x.ring = RING;
+        x.tin = tin;
+        x.tout = nullptr;
+        x.tt = tt;
+        x.c0 = c0;
+        const int xb = c0 + 64 < F->imax ? c0 + 64 : F->imax;
+        x.a = c0;
+        x.b = xb;
+        const bool exact = F->x_exact || c0 <= F->W + 12;
+        LineEnds ends;
+        __syncthreads();  // every lane has read the previous group from the tile
+        x.start(exact ? (RING ? -6 : 0) : c0 - F->W - 6);
+        filter_line<R, false, 1, XTIO<R>, RING>(x, R(F->sigma), F->imax, exact ? 0 : c0, xb, F->W, ends);
+        __syncthreads();  // the tile is complete
+        g0 = g;
+        gend = g + XY_G;
+    }
+    __device__ __forceinline__ R load(int p)
+    {
+        int r = p - g0;
+        r = r < 0 ? 0 : (r < XY_G ? r : XY_G - 1);  // clamped positions are never used
+        return tt[r][lane];
+    }
+    __device__ __forceinline__ void burst(int p)
+    {
+        if (p >= gend && p < F->jmax) fill(gend);
+    }
+    __device__ __forceinline__ void emit(int e, R v)
+    {
+        if (active && e >= a && e < b) out[(size_t)e * F->pitch + col] = corr ? R(double(v) + corr[e]) : v;
+    }
+    __device__ __forceinline__ void finish() {}
+    __device__ __forceinline__ R load_head(int) { return R(0); }
+};
+
+template <typename R, bool RING>
+__device__ __forceinline__ void filter_xy_tile(const FilterXYArgs<R> &F, int strip, int chunk, int k,
+                                               R (*tin)[XIO<R>::T + 1], R (*tt)[65])
+{
+    int a, b;
+    if (!chunk_bounds(chunk, F.Ly, F.jmax, a, b)) return;
+    const int c0 = strip * 64;
+    YTIO<R, RING> io;
+    io.F = &F;
+    io.k = k;
+    io.c0 = c0;
+    io.lane = threadIdx.x;
+    io.col = c0 + (int)threadIdx.x;
+    io.active = io.col < F.imax;
+    io.a = a;
+    io.b = b;
+    io.tt = tt;
+    io.tin = tin;
+    io.out = F.out[k];
+    io.corr = F.corr[k];
+    io.gend = 0;
+    // the first group starts at the y sweep's first load (filter_line:
+    // row 0 for the exact start, else a - W - 6)
+    const bool exact = a <= F.W + 12;
+    io.fill(exact ? 0 : a - F.W - 6);
+    LineEnds ends;
+    filter_line<R, true, 1, YTIO<R, RING>>(io, R(F.sigma), F.jmax, a, b, F.W, ends);
+    // ccpml rows j=1 and j=jmax (cc.f90:46-59), as filter_y_tile
+    const int col = io.col;
+    if (io.active && (F.xper || (col >= 1 && col <= F.imax - 2))) {
+        R *o = F.out[k];
+        const int jm = F.jmax;
+        const double *cr = F.corr[k];
+        if (a == 0) {
+            double val;
+            if (k < 3) {
+                const double v1 = cr ? ends.v1 + cr[1] : ends.v1, v2 = cr ? ends.v2 + cr[2] : ends.v2;
+                val = 2.0 * v1 - v2 + F.re.bot_res[k];
+            } else {
+                val = F.re.bot_val[k];
+            }
+            o[col] = R(val);
+        }
+        if (b == jm) {
+            double val;
+            if (k < 3) {
+                const double v2 = cr ? ends.vn2 + cr[jm - 2] : ends.vn2;
+                const double v3 = cr ? ends.vn3 + cr[jm - 3] : ends.vn3;
+                val = 2.0 * v2 - v3 + F.re.top_res[k];
+            } else {
+                val = F.re.top_val[k];
+            }
+            o[(size_t)(jm - 1) * F.pitch + col] = R(val);
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void filter_xy_kernel(FilterXYArgs<R> F)
+{
+    __shared__ R tin[64][XIO<R>::T + 1];
+    __shared__ R tt[XY_G][65];
+    if (F.xper)
+        filter_xy_tile<R, true>(F, blockIdx.x, blockIdx.y, blockIdx.z, tin, tt);
+    else
+        filter_xy_tile<R, false>(F, blockIdx.x, blockIdx.y, blockIdx.z, tin, tt);
+}
+
+template <typename S>
+static hipError_t launch_filter_xy_t(const FilterXYArgs<S> &F, hipStream_t s)
+{
+    hipError_t e = kernel_ready((const void *)filter_xy_kernel<S>, 0, 64, s);
+    if (e != hipSuccess) return e;
+    dim3 grid((F.imax + 63) / 64, (F.jmax + F.Ly - 1) / F.Ly, F.nk);
+    hipLaunchKernelGGL(filter_xy_kernel<S>, grid, dim3(64), 0, s, F);
+    return hipGetLastError();
+}
+
+#if defined(NS_KERNELS_F64)
+hipError_t launch_filter_xy(const FilterXYArgs<double> &F, hipStream_t s) { return launch_filter_xy_t(F, s); }
+#endif
+#if defined(NS_KERNELS_F32)
+hipError_t launch_filter_xy(const FilterXYArgs<float> &F, hipStream_t s) { return launch_filter_xy_t(F, s); }
+#endif
 
 template <typename S>
 static hipError_t launch_filter_t(const FilterArgs<S> &F, bool xdir, hipStream_t s)
@@ -1637,19 +1814,21 @@ static hipError_t prepare_step_t(const StageArgs<S> &A, int nk, bool visc, int v
         h.plan_only = true;
         if ((e = launch_stage(A, nk, visc, false, variant, nullptr, nullptr, &h)) != hipSuccess) return e;
     }
-    const void *kerns[3];
+    const void *kerns[4];
     if (batch) {
         kerns[0] = (const void *)filter_x_kernel_batch<S>;
         kerns[1] = (const void *)filter_y_kernel_batch<S>;
         kerns[2] = (const void *)edge_i_kernel_batch<S>;
+        kerns[3] = nullptr;
     } else {
         kerns[0] = (const void *)filter_x_kernel<S>;
         kerns[1] = (const void *)filter_y_kernel<S>;
         kerns[2] = (const void *)edge_i_kernel<S>;
+        kerns[3] = (const void *)filter_xy_kernel<S>;
     }
-    const int threads[3] = {64, 64, 256};
-    for (int q = 0; q < 3; ++q)
-        if ((e = kernel_ready(kerns[q], 0, threads[q], nullptr)) != hipSuccess) return e;
+    const int threads[4] = {64, 64, 256, 64};
+    for (int q = 0; q < 4; ++q)
+        if (kerns[q] && (e = kernel_ready(kerns[q], 0, threads[q], nullptr)) != hipSuccess) return e;
     return hipSuccess;
 }
 

@@ -297,6 +297,17 @@ class Solver:
         """Use stage variant 0..2 (see autotune) without timing."""
         _lib.check(self.lib.ns_set_variant(self.ctx, int(variant)))
 
+    def set_filter_mode(self, mode: int):
+        """0: the x and y filter sweeps as two kernels; 1: one fused sweep."""
+        _lib.check(self.lib.ns_set_filter_mode(self.ctx, int(mode)))
+
+    def filter_mode(self):
+        """(mode, [ms of the two sweeps, ms of the fused sweep]) (-1: not tuned)."""
+        m = C.c_int32()
+        ms = np.zeros(2)
+        _lib.check(self.lib.ns_get_filter_mode(self.ctx, C.byref(m), _lib.dptr(ms)))
+        return m.value, ms
+
     def enable_timing(self, on: bool = True):
         _lib.check(self.lib.ns_enable_timing(self.ctx, int(on)))
 

@@ -312,3 +312,54 @@ def test_split_rhs_entry_points():
     mety = np.where((j1 <= case.D) | (j1 >= case.jmax - case.D + 1), case.meshy, 1.0)
     assert np.array_equal(m, U * mety[:, None, None])
     s.close()
+
+
+FILTER_CASES = {
+    "small-nk5": dict(imax=150, jmax=130, nk=5, D=10, seed=61),           # lines < 272: the reference order
+    "chunks-nk4": dict(imax=700, jmax=600, nk=4, D=20, seed=62),          # look-back strips and row chunks
+    "ragged-nk5": dict(imax=333, jmax=1031, nk=5, D=12, seed=63),         # partial last strip, several chunks
+}
+
+
+@pytest.mark.parametrize("name", list(FILTER_CASES))
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fused_filter_sweep(name, periodic):
+    """The stage's filters as one fused sweep (filter mode 1) against the two
+    sweeps (mode 0) and the oracle: two steps within 1e-13 of the oracle in
+    both modes; bitwise equal to mode 0 where every line is swept from its
+    start (lines < 272 points)."""
+    case = config.random_case(**FILTER_CASES[name])
+    case.periodic_x = periodic
+    Uo = Oracle(case).run(case.U0, 0, 2)
+    out = {}
+    for mode in (0, 1):
+        s = _solver(case)
+        s.set_variant(0)
+        s.set_filter_mode(mode)
+        s.run(0, 2)
+        out[mode] = s.get_state()
+        s.close()
+        err = max(rel_l2(out[mode][..., k], Uo[..., k]) for k in range(case.nk))
+        print(f"{name} periodic={periodic} filter mode {mode}: rel L2 vs oracle {err:.3e}", flush=True)
+        assert err < 1e-13
+    if max(case.imax, case.jmax) < 272:
+        assert np.array_equal(out[0], out[1])
+
+
+def test_fused_filter_sweep_perturbation_form():
+    """FP32 perturbation form: the fused sweep (with the y filter's base
+    correction and the ccpml rows in perturbation form) against the two
+    sweeps, p' within FP32 rounding."""
+    from navierstokes_amd.solver import Solver
+    case = config.random_case(700, 600, nk=4, D=20, seed=64, amp=1e-3)
+    out = {}
+    for mode in (0, 1):
+        s = Solver(case, precision=1)
+        s.set_variant(2)
+        s.set_filter_mode(mode)
+        s.run(0, 5)
+        out[mode] = s.get_state()
+        s.close()
+    err = rel_l2(out[1][..., 3] - case.Ub[..., 3], out[0][..., 3] - case.Ub[..., 3])
+    print(f"FP32-pert fused vs two sweeps, 5 steps: p' rel L2 {err:.3e}", flush=True)
+    assert err < 1e-5
