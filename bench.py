@@ -500,6 +500,7 @@ def main():
             "pprime_rel_l2_vs_fp64": float(np.linalg.norm(r32["pprime"] - r["pprime"]) / np.linalg.norm(r["pprime"])),
             "steps_compared": r["nsteps"],
             "stage_variant": r32["variant"],
+            "stage_variant_ms": [round(x, 4) for x in r32["tune_ms"]],
             "filter_mode": r32["filter_mode"],
             "filter_tune_ms": r32["filter_tune_ms"],
             "kernel_ms": {k: round(v, 4) for k, v in kt32.items()},

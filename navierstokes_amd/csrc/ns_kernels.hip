@@ -17,6 +17,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 
 #include "ns_kernels.h"
@@ -113,6 +114,22 @@ __device__ __forceinline__ R drp_interior_unit(const F &f, int q)
     const R t2 = R(DRP_A2) * (f(q + 2) - f(q - 2));
     const R t3 = R(DRP_A3) * (f(q + 3) - f(q - 3));
     return (t1 + t2) + t3;
+}
+
+// Element g of plane p.  With a 32-bit offset the byte offset is formed in
+// 32 bits, so the compiler addresses through the plane's base in SGPRs plus a
+// 32-bit VGPR offset instead of a 64-bit address per plane and point (the
+// caller guarantees planes below 4 GiB).
+template <typename T>
+__device__ __forceinline__ T *elt(T *p, size_t g)
+{
+    return p + g;
+}
+template <typename T>
+__device__ __forceinline__ T *elt(T *p, uint32_t g)
+{
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T *>(reinterpret_cast<B *>(p) + g * (uint32_t)sizeof(T));
 }
 
 // ---------------------------------------------------------------------------
@@ -289,13 +306,13 @@ struct StorePre {
 
 // W is loaded unconditionally (gx, gy must lie in the grid) so that the
 // number of loads in flight is known to the compiler.
-template <typename S, int NK, bool PERT, int PM>
+template <typename S, int NK, bool PERT, int PM, typename IX = size_t>
 __device__ __forceinline__ void store_prefetch(const StageArgs<S> &A, int gx, int gy, StorePre<S, NK> &p)
 {
-    const size_t g = (size_t)gy * A.pitch + gx;
+    const IX g = (IX)gy * (IX)A.pitch + (IX)gx;
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
-        const S w = A.W[k][g];
+        const S w = *elt(A.W[k], g);
         p.w[k] = A.first ? S(0) : w;
     }
     if constexpr (PM != PM_FAST) {
@@ -304,22 +321,22 @@ __device__ __forceinline__ void store_prefetch(const StageArgs<S> &A, int gx, in
         p.sy = ybuf ? A.sigy[gy] : 1.0;
         p.sx = xbuf ? A.sigx[gx] : 1.0;
 #pragma unroll
-        for (int k = 0; k < NK; ++k) p.b[k] = (!PERT && (ybuf || xbuf)) ? A.base[k][g] : S(0);
+        for (int k = 0; k < NK; ++k) p.b[k] = (!PERT && (ybuf || xbuf)) ? *elt(A.base[k], g) : S(0);
     }
 }
 
 // LDDRK update at interior points (rk.f90:160-167) and the buffer layers
 // (rk.f90:81-108), or the RHS alone (RHS_ONLY).  pre: the point's data
 // loaded ahead by store_prefetch (null: load it here).
-template <typename S, typename R, int NK, bool RHS_ONLY, bool PERT, int PM>
+template <typename S, typename R, int NK, bool RHS_ONLY, bool PERT, int PM, typename IX = size_t>
 __device__ __forceinline__ void stage_store(const StageArgs<S> &A, int gx, int gy, const R *qc, const R *uc,
                                             const R *f, const StorePre<S, NK> *pre = nullptr)
 {
     const int imax = A.imax, jmax = A.jmax;
-    const size_t g = (size_t)gy * A.pitch + gx;
+    const IX g = (IX)gy * (IX)A.pitch + (IX)gx;
     if constexpr (RHS_ONLY) {
 #pragma unroll
-        for (int k = 0; k < NK; ++k) A.dudt[k][g] = S(f[k]);
+        for (int k = 0; k < NK; ++k) *elt(A.dudt[k], g) = S(f[k]);
         return;
     }
     // interior points (rk.f90:161-162; periodic x: every column)
@@ -333,29 +350,29 @@ __device__ __forceinline__ void stage_store(const StageArgs<S> &A, int gx, int g
             // stage accumulation of the state in FP64 registers
             double qn = double(qc[k]);
             if (interior) {
-                const double w0 = double(pre ? pre->w[k] : A.W[k][g]);
+                const double w0 = double(pre ? pre->w[k] : *elt(A.W[k], g));
                 const double wn = A.first ? A.dt * double(f[k]) : A.alpha * w0 + A.dt * double(f[k]);
-                if (!A.last) A.W[k][g] = S(wn);
+                if (!A.last) *elt(A.W[k], g) = S(wn);
                 qn = qn + A.beta * wn;
             }
             // Ub + s (U - Ub) - Ub = s q
             if (ybuf) qn = (pre ? pre->sy : A.sigy[gy]) * qn;
             if (xbuf) qn = (pre ? pre->sx : A.sigx[gx]) * qn;
-            A.Uout[k][g] = S(qn);
+            *elt(A.Uout[k], g) = S(qn);
         } else {
             R un = uc[k];
             if (interior) {
-                const R w0 = R(pre ? pre->w[k] : A.W[k][g]);
+                const R w0 = R(pre ? pre->w[k] : *elt(A.W[k], g));
                 const R wn = A.first ? R(A.dt) * f[k] : R(A.alpha) * w0 + R(A.dt) * f[k];
-                if (!A.last) A.W[k][g] = S(wn);
+                if (!A.last) *elt(A.W[k], g) = S(wn);
                 un = uc[k] + R(A.beta) * wn;
             }
             if (ybuf || xbuf) {  // y layer first, then x
-                const R ub = R(pre ? pre->b[k] : A.base[k][g]);
+                const R ub = R(pre ? pre->b[k] : *elt(A.base[k], g));
                 if (ybuf) un = ub + R(pre ? pre->sy : A.sigy[gy]) * (un - ub);
                 if (xbuf) un = ub + R(pre ? pre->sx : A.sigx[gx]) * (un - ub);
             }
-            A.Uout[k][g] = S(un);
+            *elt(A.Uout[k], g) = S(un);
         }
     }
 }
@@ -570,6 +587,10 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
     constexpr int NT = RB * 64, WR = MarchRing<RB>::WR, LR = MarchRing<RB>::LR;
     constexpr int NL = L1<NK, VISC>::N;
     constexpr int PF = (RB * MWX + NT - 1) / NT;  // prefetched window points per thread
+    // element offsets: 32-bit in the FP32 form (128 instead of 151 VGPRs in
+    // the fast-zone kernel, 4 waves per SIMD instead of 3; 6 % off the FP32
+    // step), 64-bit in FP64 (32-bit offsets measured 2.7 % slower there)
+    using IX = typename std::conditional<sizeof(S) == 4, uint32_t, size_t>::type;
     extern __shared__ char smem_raw[];
     R *sU = reinterpret_cast<R *>(smem_raw);  // NK x WR x MWX
     R *sT = sU + NK * WR * MWX;               // WR x MWX (VISC)
@@ -587,9 +608,9 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
     const int cmax = A.imax - 1 - ox;
     const bool xper = A.xper;
     auto load_pt = [&](int r, int c, R *v) {
-        const size_t g = (size_t)r * pitch + (xper ? wrap_col(ox + c, A.imax) : ox + (c < cmax ? c : cmax));
+        const IX g = (IX)r * (IX)pitch + (IX)(xper ? wrap_col(ox + c, A.imax) : ox + (c < cmax ? c : cmax));
 #pragma unroll
-        for (int k = 0; k < NK; ++k) v[k] = R(A.U[k][g]);
+        for (int k = 0; k < NK; ++k) v[k] = R(*elt(A.U[k], g));
     };
     auto put_pt = [&](int r, int c, const R *v) {
         const int s = (r - wy0) % WR;
@@ -695,7 +716,7 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
         }
         const R myl = MET ? R(A.mety[base + 3 + wave]) : R(1), myo = MET ? R(A.mety[base + wave]) : R(1);
         StorePre<S, NK> spre;
-        store_prefetch<S, NK, PERT, PM>(A, min(i0 + lane, A.imax - 1), base + wave, spre);
+        store_prefetch<S, NK, PERT, PM, IX>(A, min(i0 + lane, A.imax - 1), base + wave, spre);
         prefetch(base + RB + 6);
         // (b) level-1 row base+3+wave
         if constexpr (NL > 0) {
@@ -719,7 +740,7 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
             R qc[NK], uc[NK], f[NK];
             rhs_point<S, R, NK, VISC, PERT, PM>(A, gx, gy, mxo, myo, ux_at(gy), uy_at(gx, gy), lxa, lya, lv, qc,
                                                 uc, f, PERT ? &rco : nullptr);
-            stage_store<S, R, NK, false, PERT, PM>(A, gx, gy, qc, uc, f, &spre);
+            stage_store<S, R, NK, false, PERT, PM, IX>(A, gx, gy, qc, uc, f, &spre);
         }
     }
 }
@@ -776,6 +797,8 @@ static bool march_plan(const StageArgs<S> &A, MarchMap &m, int &nchunks, bool &o
     constexpr bool FAST = PM == PM_FAST;
     e = hipSuccess;
     if ((FAST || PM == PM_HYBRID) && !A.fast_ok) return false;
+    // the FP32 marching kernel forms element offsets in 32 bits
+    if (sizeof(S) == 4 && (size_t)A.pitch * (size_t)A.jmax * sizeof(S) >= ((size_t)1 << 32)) return false;
     if constexpr (PM == PM_HYBRID) {
         if (!march_hybrid(A.D, A.imax, A.jmax, RB, m, A.xper)) return false;
     } else {
@@ -1632,15 +1655,22 @@ __device__ __forceinline__ void filter_xy_tile(const FilterXYArgs<R> &F, int str
     }
 }
 
+// One block per (64-column strip, row chunk, component), numbered like the
+// marching blocks (march_tile_of): each XCD runs a contiguous run of tiles,
+// strips minor, so neighbouring strips, whose x recurrences re-read each
+// other's W + 6 look-back columns, share that XCD's L2.
 template <typename R>
-__global__ __launch_bounds__(64) void filter_xy_kernel(FilterXYArgs<R> F)
+__global__ __launch_bounds__(64) void filter_xy_kernel(FilterXYArgs<R> F, int nstrips, int nchunks)
 {
     __shared__ R tin[64][XIO<R>::T + 1];
     __shared__ R tt[XY_G][65];
+    int strip, ck;
+    march_tile_of((int)blockIdx.x, (int)gridDim.x, nstrips, strip, ck);
+    const int chunk = ck % nchunks, k = ck / nchunks;
     if (F.xper)
-        filter_xy_tile<R, true>(F, blockIdx.x, blockIdx.y, blockIdx.z, tin, tt);
+        filter_xy_tile<R, true>(F, strip, chunk, k, tin, tt);
     else
-        filter_xy_tile<R, false>(F, blockIdx.x, blockIdx.y, blockIdx.z, tin, tt);
+        filter_xy_tile<R, false>(F, strip, chunk, k, tin, tt);
 }
 
 template <typename S>
@@ -1648,8 +1678,8 @@ static hipError_t launch_filter_xy_t(const FilterXYArgs<S> &F, hipStream_t s)
 {
     hipError_t e = kernel_ready((const void *)filter_xy_kernel<S>, 0, 64, s);
     if (e != hipSuccess) return e;
-    dim3 grid((F.imax + 63) / 64, (F.jmax + F.Ly - 1) / F.Ly, F.nk);
-    hipLaunchKernelGGL(filter_xy_kernel<S>, grid, dim3(64), 0, s, F);
+    const int nstrips = (F.imax + 63) / 64, nchunks = (F.jmax + F.Ly - 1) / F.Ly;
+    hipLaunchKernelGGL(filter_xy_kernel<S>, dim3(nstrips * nchunks * F.nk), dim3(64), 0, s, F, nstrips, nchunks);
     return hipGetLastError();
 }
 

@@ -150,7 +150,7 @@ int main()
     for (int imax : sizes)
         for (int jmax : sizes)
             for (int D : Ds)
-                for (int rb : {4, 8})
+                for (int rb : {4, 6, 8})
                     for (int mode = 0; mode < 3; ++mode)
                         for (int H : {rb, 4 * rb, 7 * rb, 16 * rb, 50 * rb, 4096})
                             for (bool xper : {false, true})
