@@ -1199,6 +1199,47 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
 #pragma unroll
         for (int u = 0; u < 13; ++u) nn[u] = ld(q + 20 + u);
     }
+    // the next burst of loads (positions q + 13 PD + 7 ..) into nb / nn; cur:
+    // the burst this block of 13 steps consumes
+    auto refill = [&](R (&cur)[13]) {
+#pragma unroll
+        for (int u = 0; u < 13; ++u) cur[u] = nb[u];
+        const int qn = q + 13 * PD;
+        if constexpr (PD == 2) {
+#pragma unroll
+            for (int u = 0; u < 13; ++u) nb[u] = nn[u];
+#pragma unroll
+            for (int u = 0; u < 13; ++u) nn[u] = ld(qn + 7 + u);
+        } else {
+            io.burst(qn + 7);
+#pragma unroll
+            for (int u = 0; u < 13; ++u) nb[u] = ld(qn + 7 + u);
+        }
+    };
+    // Whole blocks of 13 steps that end before qe need no exit test per
+    // step: the warm-up blocks emit nothing (every position they finish lies
+    // before a), the block that reaches a emits through the checked path,
+    // and the blocks after it emit unchecked (positions in [a, b) there).
+    // The same update, in the same order, as the general loop below.
+    auto bulk = [&](auto mode) {
+        constexpr int M = decltype(mode)::value;  // 0: no emits, 1: checked, 2: unchecked
+        R cur[13];
+        refill(cur);
+#pragma unroll
+        for (int u = 0; u < 13; ++u) {
+            R Du = R(filt_d(0)) * w[u % 13];
+#pragma unroll
+            for (int s = 1; s < 13; ++s) Du = Du + R(filt_d(s)) * w[(u + s) % 13];
+            w[(u + 6) % 13] = w[(u + 6) % 13] - sigma * Du;
+            if constexpr (M == 1) io.emit(q - 6 + u, w[u % 13]);
+            if constexpr (M == 2) io.emit_fast(q - 6 + u, w[u % 13]);
+            w[u % 13] = cur[u];
+        }
+        q += 13;
+    };
+    while (q + 13 <= qe && q + 6 < a) bulk(std::integral_constant<int, 0>{});
+    if (q + 13 <= qe && q - 6 < a) bulk(std::integral_constant<int, 1>{});
+    while (q + 13 <= qe) bulk(std::integral_constant<int, 2>{});
     int rot = 0;
     for (;;) {
         R cur[13];
@@ -1369,6 +1410,7 @@ struct XIO {
         }
         tout[lane][e - oc] = v;
     }
+    __device__ __forceinline__ void emit_fast(int e, R v) { emit(e, v); }
     __device__ __forceinline__ void finish() { flush(); }
     __device__ __forceinline__ void burst(int) {}
     __device__ __forceinline__ R load_head(int p)  // this lane's row, position p (unstaged)
@@ -1427,7 +1469,9 @@ __global__ __launch_bounds__(64) void filter_x_kernel_batch(const FilterArgs<R> 
 }
 
 // y direction: lanes are columns; rows are contiguous across lanes.
-template <typename R>
+// CORR: the perturbation form adds Ly(Ub) - Ub of the row (corr, FP64);
+// FULL: every lane's column lies in the grid
+template <typename R, bool CORR, bool FULL>
 struct YIO {
     const R *in;
     R *out;
@@ -1436,22 +1480,29 @@ struct YIO {
     bool active;
     int colc;  // col clamped into the grid: inactive lanes load a valid value and ignore it
     __device__ __forceinline__ R load(int p) { return in[(size_t)p * pitch + colc]; }
+    __device__ __forceinline__ R corrected(int e, R v) const
+    {
+        if constexpr (CORR) return R(double(v) + corr[e]);
+        else return v;
+    }
     __device__ __forceinline__ void emit(int e, R v)
     {
-        if (active && e >= a && e < b) out[(size_t)e * pitch + col] = corr ? R(double(v) + corr[e]) : v;
+        if ((FULL || active) && e >= a && e < b) out[(size_t)e * pitch + col] = corrected(e, v);
+    }
+    // e in [a, b)
+    __device__ __forceinline__ void emit_fast(int e, R v)
+    {
+        if (FULL || active) out[(size_t)e * pitch + col] = corrected(e, v);
     }
     __device__ __forceinline__ void finish() {}
     __device__ __forceinline__ void burst(int) {}
     __device__ __forceinline__ R load_head(int p) { return in[(size_t)p * pitch + colc]; }
 };
 
-template <typename R>
-__device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblock, int chunk, int k)
+template <typename R, bool CORR, bool FULL>
+__device__ __forceinline__ void filter_y_lines(const FilterArgs<R> &F, int col, int k, int a, int b, LineEnds &ends)
 {
-    int a, b;
-    if (!chunk_bounds(chunk, F.L, F.n, a, b)) return;
-    const int col = colblock * 64 + threadIdx.x;
-    YIO<R> io;
+    YIO<R, CORR, FULL> io;
     io.in = F.in[k];
     io.out = F.out[k];
     io.corr = F.corr[k];
@@ -1461,11 +1512,28 @@ __device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblo
     io.a = a;
     io.b = b;
     io.active = col < F.m;
-    LineEnds ends;
     filter_line<R, true, 2>(io, R(F.sigma), F.n, a, b, F.W, ends);
+}
+
+template <typename R>
+__device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblock, int chunk, int k)
+{
+    int a, b;
+    if (!chunk_bounds(chunk, F.L, F.n, a, b)) return;
+    const int col = colblock * 64 + threadIdx.x;
+    LineEnds ends;
+    const bool full = colblock * 64 + 64 <= F.m;
+    if (F.corr[k]) {
+        if (full) filter_y_lines<R, true, true>(F, col, k, a, b, ends);
+        else filter_y_lines<R, true, false>(F, col, k, a, b, ends);
+    } else {
+        if (full) filter_y_lines<R, false, true>(F, col, k, a, b, ends);
+        else filter_y_lines<R, false, false>(F, col, k, a, b, ends);
+    }
+    const bool active = col < F.m;
     // ccpml rows j=1 and j=jmax for columns 2..imax-1 (cc.f90:46-59; periodic
     // x: every column, there are no inflow / outflow columns)
-    if (F.edge_bc && io.active && (F.xper || (col >= 1 && col <= F.m - 2))) {
+    if (F.edge_bc && active && (F.xper || (col >= 1 && col <= F.m - 2))) {
         R *o = F.out[k];
         const int jm = F.n;
         const double *cr = F.corr[k];
@@ -1537,6 +1605,7 @@ struct XTIO : XIO<R> {
     {
         if (e >= c0 && e < c0 + 64 && this->lane < XY_G) tt[this->lane][e - c0] = v;
     }
+    __device__ __forceinline__ void emit_fast(int e, R v) { emit(e, v); }
     __device__ __forceinline__ void finish() {}
 };
 
@@ -1594,6 +1663,7 @@ struct YTIO {
     {
         if (active && e >= a && e < b) out[(size_t)e * F->pitch + col] = corr ? R(double(v) + corr[e]) : v;
     }
+    __device__ __forceinline__ void emit_fast(int e, R v) { emit(e, v); }
     __device__ __forceinline__ void finish() {}
     __device__ __forceinline__ R load_head(int) { return R(0); }
 };
