@@ -281,9 +281,10 @@ int32_t ns_set_variant(ns_ctx *ctx, int32_t variant);
    the y recurrence).  Both start each chunk's recurrences W points early
    from unfiltered values (exact to 1e-19 relative), so they agree to
    rounding; lines shorter than 272 points are swept in the reference's
-   order by both (bitwise).  ns_autotune picks the faster mode unless this
-   call pinned one; ns_get_filter_mode reports the mode and (ms[2], may be
-   NULL) the tuned times of both (-1: not tuned).  No reference
+   order by both (bitwise).  ns_autotune picks the faster mode, each timed
+   after the stage kernel as in a step, unless this call pinned one;
+   ns_get_filter_mode reports the mode and (ms[2], may be NULL) the tuned
+   times of a stage followed by each form (-1: not tuned).  No reference
    counterpart. */
 int32_t ns_set_filter_mode(ns_ctx *ctx, int32_t mode);
 int32_t ns_get_filter_mode(ns_ctx *ctx, int32_t *mode, double *ms);

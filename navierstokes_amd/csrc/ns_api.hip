@@ -618,8 +618,31 @@ int autotune_stage_t(ns_ctx *c)
             }
         }
         c->filt_Ly = bestLy;
-        c->filt_mode = bestxy < 0.97 * bestL_ms ? 1 : 0;
-        c->filt_xy_ms = bestxy;
+        c->filt_mode = mode0;
+        // the choice: each form timed after the chosen stage variant, as in
+        // a step (what the stage leaves in the caches changes the filters'
+        // time: in isolation the fused sweep can win where it loses in the
+        // step); the fused form must be more than 3 % faster
+        auto stage_then = [&](int mode) -> hipError_t {
+            c->filt_mode = mode;
+            hipError_t e = launch(best);
+            if (e == hipSuccess) e = launch_filters<S>(c, S1, S2, S2, c->pert, c->stream);
+            return e;
+        };
+        double form_ms[2];
+        for (int mode = 0; mode < 2; ++mode) {
+            HIPCHK(stage_then(mode));
+            HIPCHK(hipEventRecord(c->ev[0], c->stream));
+            for (int r = 0; r < reps; ++r) HIPCHK(stage_then(mode));
+            HIPCHK(hipEventRecord(c->ev[1], c->stream));
+            HIPCHK(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            form_ms[mode] = ms / reps;
+        }
+        c->filt_mode = form_ms[1] < 0.97 * form_ms[0] ? 1 : 0;
+        c->filt_ms = form_ms[0];
+        c->filt_xy_ms = form_ms[1];
     }
     return NS_OK;
 }

@@ -1331,14 +1331,15 @@ __device__ __forceinline__ void filter_line(IO &io, R sigma, int n, int a, int b
     io.finish();
 }
 
-// x direction: lanes are rows; LDS-staged 64x16 tiles keep global access
-// coalesced (lanes read/write 16-column row segments).
-// Tiles are 64 rows x 16 columns (one 128-byte line per row segment in FP64;
-// 32-column FP32 tiles would fill lines too, but their 32 in-flight loads per
-// lane raise the register count enough to cost more occupancy than they save).
-template <typename R>
+// x direction: lanes are rows; LDS-staged 64 x TW tiles keep global access
+// coalesced (lanes read/write TW-column row segments).  The x sweep takes
+// one 128-byte line per row segment (16 columns in FP64, 32 in FP32: the
+// 64-byte FP32 segments fetched each line twice, FP32 x sweep 0.172 ->
+// 0.152 ms at 4096^2); the fused sweep's x phase keeps 16 columns (its
+// registers do not allow 32).
+template <typename R, int TW = 16>
 struct XIO {
-    static constexpr int T = 16;
+    static constexpr int T = TW;
     static constexpr int RP = 64 / T;  // rows per pass
     const R *in;
     R *out;
@@ -1421,14 +1422,17 @@ struct XIO {
 };
 
 template <typename R>
+using XSweepIO = XIO<R, 128 / sizeof(R)>;
+
+template <typename R>
 __device__ __forceinline__ void filter_x_tile(const FilterArgs<R> &F, int chunk, int rowblock, int k)
 {
-    constexpr int T = XIO<R>::T;
+    constexpr int T = XSweepIO<R>::T;
     __shared__ R tin[64][T + 1];
     __shared__ R tout[64][T + 1];
     int a, b;
     if (!chunk_bounds(chunk, F.L, F.n, a, b)) return;
-    XIO<R> io;
+    XSweepIO<R> io;
     io.in = F.in[k];
     io.out = F.out[k];
     io.pitch = F.pitch;
@@ -1445,7 +1449,7 @@ __device__ __forceinline__ void filter_x_tile(const FilterArgs<R> &F, int chunk,
     LineEnds ends;
     if (F.xper) {
         io.start(exact ? -6 : a - F.W - 6);
-        filter_line<R, false, 1, XIO<R>, true>(io, R(F.sigma), F.n, a, b, F.W, ends);
+        filter_line<R, false, 1, XSweepIO<R>, true>(io, R(F.sigma), F.n, a, b, F.W, ends);
     } else {
         io.start(exact ? 0 : a - F.W - 6);
         filter_line<R, false, 1>(io, R(F.sigma), F.n, a, b, F.W, ends);
@@ -1520,7 +1524,7 @@ __device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblo
 {
     int a, b;
     if (!chunk_bounds(chunk, F.L, F.n, a, b)) return;
-    const int col = colblock * 64 + threadIdx.x;
+    const int col = colblock * 64 + (int)(threadIdx.x & 63);
     LineEnds ends;
     const bool full = colblock * 64 + 64 <= F.m;
     if (F.corr[k]) {
@@ -1561,10 +1565,23 @@ __device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblo
     }
 }
 
+// y sweep blocks: FP32 rows are 256 bytes per 64-column wave, so a block
+// takes 4 waves on adjacent column blocks, which walk down the same rows
+// together (1 KB of each row at a time: FP32 y sweep 0.128 -> 0.116 ms at
+// 4096^2); FP64 rows are 512 bytes per wave already, and 4-wave blocks
+// measured slower there
 template <typename R>
-__global__ __launch_bounds__(64) void filter_y_kernel(FilterArgs<R> F)
+constexpr int filter_y_waves()
 {
-    filter_y_tile<R>(F, blockIdx.x, blockIdx.y, blockIdx.z);
+    return sizeof(R) == 4 ? 4 : 1;
+}
+
+template <typename R>
+__global__ __launch_bounds__(64 * filter_y_waves<R>()) void filter_y_kernel(FilterArgs<R> F)
+{
+    constexpr int NW = filter_y_waves<R>();
+    const int cb = (int)blockIdx.x * NW + (int)(threadIdx.x >> 6);
+    if (cb * 64 < F.m) filter_y_tile<R>(F, cb, blockIdx.y, blockIdx.z);
 }
 
 // batched: tiles[blockIdx.x] = (case, column block, chunk), blockIdx.y = variable
@@ -1764,14 +1781,16 @@ template <typename S>
 static hipError_t launch_filter_t(const FilterArgs<S> &F, bool xdir, hipStream_t s)
 {
     const int nchunks = (F.n + F.L - 1) / F.L;
-    hipError_t e = kernel_ready(xdir ? (const void *)filter_x_kernel<S> : (const void *)filter_y_kernel<S>, 0, 64, s);
+    hipError_t e = kernel_ready(xdir ? (const void *)filter_x_kernel<S> : (const void *)filter_y_kernel<S>, 0,
+                                xdir ? 64 : 64 * filter_y_waves<S>(), s);
     if (e != hipSuccess) return e;
     if (xdir) {
         dim3 grid(nchunks, (F.m + 63) / 64, F.nk);
         hipLaunchKernelGGL(filter_x_kernel<S>, grid, dim3(64), 0, s, F);
     } else {
-        dim3 grid((F.m + 63) / 64, nchunks, F.nk);
-        hipLaunchKernelGGL(filter_y_kernel<S>, grid, dim3(64), 0, s, F);
+        constexpr int NW = filter_y_waves<S>();
+        dim3 grid((F.m + 64 * NW - 1) / (64 * NW), nchunks, F.nk);
+        hipLaunchKernelGGL(filter_y_kernel<S>, grid, dim3(64 * NW), 0, s, F);
     }
     return hipGetLastError();
 }
@@ -1926,7 +1945,7 @@ static hipError_t prepare_step_t(const StageArgs<S> &A, int nk, bool visc, int v
         kerns[2] = (const void *)edge_i_kernel<S>;
         kerns[3] = (const void *)filter_xy_kernel<S>;
     }
-    const int threads[4] = {64, 64, 256, 64};
+    const int threads[4] = {64, batch ? 64 : 64 * filter_y_waves<S>(), 256, 64};
     for (int q = 0; q < 4; ++q)
         if (kerns[q] && (e = kernel_ready(kerns[q], 0, threads[q], nullptr)) != hipSuccess) return e;
     return hipSuccess;
