@@ -7,6 +7,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 namespace nsamd {
@@ -122,6 +123,106 @@ __device__ __forceinline__ R drp_deriv(const F &f, int q, int n, R inv_h)
     return drp_right_row<2, R>(f, n, inv_h);
 }
 
+// Natural logarithm and exponential for the transport coefficients (FP64).
+// The arguments are bounded (a positive normal temperature times Tref; an
+// exponent of a few tens), so neither needs the special-case handling of a
+// general library routine, which costs the FP64 log about 100 and the exp
+// about 45 instructions.  Both are textbook methods with explicit FMAs,
+// within 1.5 ulp (log) and 1 ulp (exp) of the correctly rounded result
+// (tests/transport_math_check.cpp measures them against the C library).
+//
+// ln2 split (Cody & Waite): LN2_HI keeps a 32-bit significand, so e * LN2_HI
+// is exact for |e| < 2^21; LN2_HI + LN2_LO = ln 2 to 1e-26.
+constexpr double NS_LN2_HI = 0.6931471803691238;
+constexpr double NS_LN2_LO = 1.9082149292705877e-10;
+constexpr double NS_INV_LN2 = 1.4426950408889634;
+
+// 1 / d for d in [1.7, 2.5): a hardware estimate (the host: the division)
+// refined by two Newton steps
+__host__ __device__ inline double ns_recip(double d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+#else
+    double r = 1.0 / d;
+#endif
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
+// log x for a positive normal x: x = 2^e m with m in [sqrt(1/2), sqrt(2)),
+// log m = 2 atanh(s), s = (m - 1)/(m + 1), |s| < 0.1716, and
+// 2 atanh(s) = 2s + 2s (s^2/3 + s^4/5 + ... + s^18/19) (the next term is
+// below 3e-17 of the sum)
+__host__ __device__ inline double ns_log(double x)
+{
+    int e;
+    double m = frexp(x, &e);  // m in [0.5, 1)
+    if (m < 0.70710678118654752) {
+        m = m + m;
+        e = e - 1;
+    }
+    const double f = m - 1.0;         // exact
+    const double d = m + 1.0;         // rounded: m + 1 = d + dl exactly,
+    const double dl = m - (d - 1.0);  // both differences exact (Sterbenz)
+    const double r = ns_recip(d);
+    double s = f * r;
+    s = fma(fma(-s, dl, fma(-s, d, f)), r, s);  // the residual f - s (d + dl) folded back
+    const double z = s * s;
+    double p = 1.0 / 19.0;
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    const double s2 = s + s;
+    const double lm = fma(s2 * z, p, s2);
+    const double de = (double)e;
+    return fma(de, NS_LN2_HI, fma(de, NS_LN2_LO, lm));
+}
+
+// exp a for |a| < 700: a = n ln2 + r, |r| <= ln2/2, e^r by its Taylor
+// polynomial through r^13 (the remainder is below 5e-18 of e^r)
+__host__ __device__ inline double ns_exp(double a)
+{
+    const double n = rint(a * NS_INV_LN2);
+    double r = fma(-n, NS_LN2_HI, a);
+    r = fma(-n, NS_LN2_LO, r);
+    double p = 1.0 / 6227020800.0;          // 1/13!
+    p = fma(p, r, 1.0 / 479001600.0);       // 1/12!
+    p = fma(p, r, 1.0 / 39916800.0);        // 1/11!
+    p = fma(p, r, 1.0 / 3628800.0);         // 1/10!
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
+template <typename R>
+__device__ __forceinline__ R tr_log(R x)
+{
+    if constexpr (sizeof(R) == 8) return ns_log(x);
+    else return log(x);
+}
+template <typename R>
+__device__ __forceinline__ R tr_exp(R x)
+{
+    if constexpr (sizeof(R) == 8) return ns_exp(x);
+    else return exp(x);
+}
+
 // Transport coefficients, coefficients.f90:35-42 and :50-58:
 //   mu = exp(A ln(Tref T) + B/(Tref T) + C/(Tref T)^2 + D) / MI0 (same for k).
 // B/Tref, C/Tref^2 and 1/MI0, 1/K0 are folded on the host; the result agrees
@@ -136,10 +237,10 @@ struct Transport {
 template <typename R>
 __device__ __forceinline__ void transport_eval(const Transport<R> &tc, R T, R &mi, R &kk)
 {
-    const R lt = log(tc.Tref * T);
+    const R lt = tr_log<R>(tc.Tref * T);
     const R it = R(1) / T;
-    mi = tc.inv_mi0 * exp(tc.A_mi * lt + tc.Bt_mi * it + tc.Ct_mi * it * it + tc.D_mi);
-    kk = tc.inv_k0 * exp(tc.A_k * lt + tc.Bt_k * it + tc.Ct_k * it * it + tc.D_k);
+    mi = tc.inv_mi0 * tr_exp<R>(tc.A_mi * lt + tc.Bt_mi * it + tc.Ct_mi * it * it + tc.D_mi);
+    kk = tc.inv_k0 * tr_exp<R>(tc.A_k * lt + tc.Bt_k * it + tc.Ct_k * it * it + tc.D_k);
 }
 
 }  // namespace nsamd
