@@ -210,17 +210,21 @@ __host__ __device__ inline double ns_exp(double a)
     return ldexp(p, (int)n);
 }
 
+// FP32 (the perturbation form): the hardware base-2 log and exp.  Their
+// result differs from a correctly rounded one by a few parts in 1e7 for the
+// exponents the coefficients take (|a| of order 10), far inside the form's
+// 1e-4 error bar; mu and k only scale the viscous terms.
 template <typename R>
 __device__ __forceinline__ R tr_log(R x)
 {
     if constexpr (sizeof(R) == 8) return ns_log(x);
-    else return log(x);
+    else return __builtin_amdgcn_logf(x) * 0.69314718f;
 }
 template <typename R>
 __device__ __forceinline__ R tr_exp(R x)
 {
     if constexpr (sizeof(R) == 8) return ns_exp(x);
-    else return exp(x);
+    else return __builtin_amdgcn_exp2f(x * 1.44269504f);
 }
 
 // Transport coefficients, coefficients.f90:35-42 and :50-58:
@@ -238,7 +242,9 @@ template <typename R>
 __device__ __forceinline__ void transport_eval(const Transport<R> &tc, R T, R &mi, R &kk)
 {
     const R lt = tr_log<R>(tc.Tref * T);
-    const R it = R(1) / T;
+    R it;
+    if constexpr (sizeof(R) == 8) it = R(1) / T;
+    else it = __builtin_amdgcn_rcpf(T);  // the hardware reciprocal, 1 ulp
     mi = tc.inv_mi0 * tr_exp<R>(tc.A_mi * lt + tc.Bt_mi * it + tc.Ct_mi * it * it + tc.D_mi);
     kk = tc.inv_k0 * tr_exp<R>(tc.A_k * lt + tc.Bt_k * it + tc.Ct_k * it * it + tc.D_k);
 }
