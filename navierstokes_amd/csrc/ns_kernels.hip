@@ -756,8 +756,18 @@ __device__ __forceinline__ void march_tile(int nstrips, int &strip, int &chunk)
                   chunk);
 }
 
+// FP32 blocks of 4 rows per step take 36 KB of LDS (nk = 4), so 4 fit a
+// CU: the registers must allow 4 waves per SIMD as well (the general-region
+// kernel then keeps 128 VGPRs instead of 137; nk = 5 would spill)
+template <typename S, int RB, int NK>
+constexpr int march_waves_per_simd()
+{
+    return (sizeof(S) == 4 && RB == 4 && NK == 4) ? 4 : 1;
+}
+
 template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
-__global__ __launch_bounds__(RB * 64) void stage_march_kernel(StageArgs<S> A, MarchMap m)
+__global__ __launch_bounds__(RB * 64, (march_waves_per_simd<S, RB, NK>())) void stage_march_kernel(StageArgs<S> A,
+                                                                                           MarchMap m)
 {
     int strip, chunk;
     march_tile(m.nstrips, strip, chunk);
