@@ -17,6 +17,8 @@
 // perturbation form (S = float, U = Ub(j) + q).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ns_adjoint.h"
 
 namespace nsamd {
@@ -112,6 +114,35 @@ __host__ __device__ __forceinline__ void adj_filter_line(const LineRef<const S> 
         }
     }
     // I^T from q down to qs.  At step u position q+s sits in slot (6+s-u) mod 13.
+    // Whole blocks of 13 steps that stop above qs need no exit test per
+    // step, and their 13 loads can all be issued up front: the blocks whose
+    // emitted positions (q+6 .. q-6) all lie at or above b emit nothing, the
+    // block that reaches b emits through the bounds test, the blocks after it
+    // emit unchecked.  The same operations, in the same order, as the
+    // general loop below.
+    auto bulk = [&](auto mode) {
+        constexpr int M = decltype(mode)::value;  // 0: no emits, 1: checked, 2: unchecked
+#pragma unroll
+        for (int u = 0; u < 13; ++u) {
+            const S t = sigma * w[(6 - u + 13) % 13];
+#pragma unroll
+            for (int s = -6; s <= 6; ++s) {
+                const int sl = (6 + s - u + 26) % 13;
+                w[sl] = w[sl] - S(filt_d(s + 6)) * t;
+            }
+            const int sl = (12 - u + 13) % 13;  // position q-u+6, final now
+            const int e = q - u + 6;
+            if constexpr (M == 1) {
+                if (e >= a && e < b) Out(e) = w[sl];
+            }
+            if constexpr (M == 2) Out(e) = w[sl];
+            w[sl] = In(q - u - 7);
+        }
+        q -= 13;
+    };
+    while (q - 13 >= qs && q - 6 >= b) bulk(std::integral_constant<int, 0>{});
+    if (q - 13 >= qs && q + 6 >= b) bulk(std::integral_constant<int, 1>{});
+    while (q - 13 >= qs && q - 6 >= a) bulk(std::integral_constant<int, 2>{});
     int rot = 0;
     for (;;) {
         bool done = false;
