@@ -157,7 +157,8 @@ struct ns_ctx {
     double **dptrs = nullptr;  // device pointer tables for operator kernels
     int *dflag = nullptr;
     bool have_profiles = false, have_base = false, have_state = false;
-    int filt_L = 256, filt_W = 40;
+    int filt_L = 256, filt_W = 40;  // y sweep chunk length, look-back
+    int filt_Lx = 256;              // x sweep chunk length
     double filt_ms = -1.0;  // autotuned time of both filter sweeps at filt_L (-1: not tuned)
     int filt_mode = 0;      // 0: x and y sweeps as two kernels; 1: one fused sweep (filter_xy_kernel)
     double filt_xy_ms = -1.0;  // autotuned time of the fused sweep (-1: not tuned)
@@ -346,7 +347,7 @@ FilterArgs<S> filter_args(ns_ctx *c, bool xdir, S *in, S *out, bool edge_bc, boo
     F.n = xdir ? P.imax : P.jmax;
     F.m = xdir ? P.jmax : P.imax;
     F.pitch = c->pitch;
-    F.L = c->filt_L;
+    F.L = xdir ? c->filt_Lx : c->filt_L;
     F.W = c->filt_W;
     F.sigma = P.sigmaf;
     F.nk = P.nk;
@@ -556,37 +557,48 @@ int autotune_stage_t(ns_ctx *c)
         }
     }
     c->stage_variant = best;
-    // look-back chunk length of the filters (more chunks: more waves in
+    // look-back chunk length of each filter sweep (more chunks: more waves in
     // flight; longer chunks: less warm-up work), timed on the scratch buffers
-    // S1 -> S2 -> S1 so the state is untouched.  Lines shorter than 272
-    // points keep L = 256: one chunk, the exact sequential sweep, so small
-    // grids stay bitwise equal to the reference order (a look-back start is
-    // exact to 1e-19 relative, which can still flip a rounding now and then).
+    // S1 -> S2 (x) and S2 -> S1 (y) so the state is untouched.  Lines
+    // shorter than 272 points keep L = 256: one chunk, the exact sequential
+    // sweep, so small grids stay bitwise equal to the reference order (a
+    // look-back start is exact to 1e-19 relative, which can still flip a
+    // rounding now and then).
     S *S2 = static_cast<S *>(c->S[2]);
-    const int Ls[3] = {128, 256, 512};
-    int bestL = c->filt_L;
-    double bestL_ms = 1e300;
-    const int ntune = std::max(P.imax, P.jmax) < 272 ? 0 : 3;
-    for (int t = 0; t < ntune; ++t) {
-        c->filt_L = Ls[t];
-        auto sweep = [&]() -> hipError_t {
-            hipError_t e = launch_filter(filter_args<S>(c, true, S1, S2, false, c->pert), true, c->stream);
-            if (e == hipSuccess) e = launch_filter(filter_args<S>(c, false, S2, S1, true, c->pert), false, c->stream);
-            return e;
-        };
-        HIPCHK(sweep());
-        HIPCHK(hipEventRecord(c->ev[0], c->stream));
-        for (int r = 0; r < reps; ++r) HIPCHK(sweep());
-        HIPCHK(hipEventRecord(c->ev[1], c->stream));
-        HIPCHK(hipEventSynchronize(c->ev[1]));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-        if (ms / reps < bestL_ms) {
-            bestL_ms = ms / reps;
-            bestL = Ls[t];
+    const int Ls[4] = {128, 256, 512, 1024};
+    auto tune_L = [&](bool xdir, int &L, double &best_ms) -> int {
+        const int n = xdir ? P.imax : P.jmax;
+        if (n < 272) return NS_OK;
+        int bestL = L;
+        best_ms = 1e300;
+        for (int t = 0; t < 4; ++t) {
+            if (Ls[t] >= n && t > 0) break;
+            L = Ls[t];
+            auto sweep = [&]() -> hipError_t {
+                return xdir ? launch_filter(filter_args<S>(c, true, S1, S2, false, c->pert), true, c->stream)
+                            : launch_filter(filter_args<S>(c, false, S2, S1, true, c->pert), false, c->stream);
+            };
+            HIPCHK(sweep());
+            HIPCHK(hipEventRecord(c->ev[0], c->stream));
+            for (int r = 0; r < reps; ++r) HIPCHK(sweep());
+            HIPCHK(hipEventRecord(c->ev[1], c->stream));
+            HIPCHK(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            if (ms / reps < best_ms) {
+                best_ms = ms / reps;
+                bestL = Ls[t];
+            }
         }
-    }
-    c->filt_L = bestL;
+        L = bestL;
+        return NS_OK;
+    };
+    double x_ms = 0.0, y_ms = 0.0;
+    int rc;
+    if ((rc = tune_L(true, c->filt_Lx, x_ms))) return rc;
+    if ((rc = tune_L(false, c->filt_L, y_ms))) return rc;
+    const int ntune = std::max(P.imax, P.jmax) < 272 ? 0 : 3;
+    const double bestL_ms = x_ms + y_ms;
     if (ntune) c->filt_ms = bestL_ms;
     // the fused sweep (filter_xy_kernel) against the two sweeps, with its
     // row chunk length; the fused one must be more than 3 % faster to be
@@ -1810,7 +1822,7 @@ int batch_prepare_t(ns_batch *b)
         fx[c] = filter_args<S>(x, true, S1, S2, false, b->pert);
         fy[c] = filter_args<S>(x, false, S2, S0, true, b->pert);
         ea[c] = edge_args<S>(x, S0, b->pert);
-        const int imax = x->P.imax, jmax = x->P.jmax, L = x->filt_L;
+        const int imax = x->P.imax, jmax = x->P.jmax, L = x->filt_L, Lx = x->filt_Lx;
         int fx0, fx1, fy0, fy1;
         const StageArgs<S> &A0 = sa[c];
         fast_range(A0.fast_ok, A0.D, imax, 64, fx0, fx1);
@@ -1821,7 +1833,7 @@ int batch_prepare_t(ns_batch *b)
                 (fast ? st : gt).push_back(make_int4(c, bx, by, 0));
             }
         for (int rb = 0; rb < (jmax + 63) / 64; ++rb)
-            for (int ch = 0; ch < (imax + L - 1) / L; ++ch) tx.push_back(make_int4(c, ch, rb, 0));
+            for (int ch = 0; ch < (imax + Lx - 1) / Lx; ++ch) tx.push_back(make_int4(c, ch, rb, 0));
         for (int ch = 0; ch < (jmax + L - 1) / L; ++ch)
             for (int cb = 0; cb < (imax + 63) / 64; ++cb) ty.push_back(make_int4(c, cb, ch, 0));
     }
