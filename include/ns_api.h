@@ -297,7 +297,8 @@ int32_t ns_get_filter_mode(ns_ctx *ctx, int32_t *mode, double *ms);
 int32_t ns_enable_timing(ns_ctx *ctx, int32_t on);
 int32_t ns_kernel_time(ns_ctx *ctx, const char *name, double *avg_ms, int32_t *count);
 /* output points covered by the stage's interior kernel in the last timed
-   stage (the rest of the grid runs as edge tiles).
+   stage (the rest of the grid runs as edge tiles; the whole grid when the
+   edge tiles run in the interior kernel's launch).
    No reference counterpart: measurement only. */
 int32_t ns_stage_interior(ns_ctx *ctx, int64_t *points);
 

@@ -166,6 +166,7 @@ struct ns_ctx {
     bool filt_pinned = false;  // ns_set_filter_mode chose the mode: the autotuner keeps it
     hipGraphExec_t graph = nullptr;
     int stage_variant = -1;       // block-size variant of the stage kernel (-1: not tuned)
+    bool same_launch = false;     // marching stages: edge tiles in the marching launch (autotuned)
     double tune_ms[3] = {0, 0, 0};
     bool timing = false;
     hipEvent_t ev[2 * 24] = {};
@@ -438,19 +439,18 @@ int enqueue_stage_t(ns_ctx *c, int irk, int iter, bool timed, S *in = nullptr, S
     };
     StageArgs<S> A = stage_args<S>(c, irk, S0, S1, stage_amp(P, irk, iter), pert);
     tick(0);
-    // interior tiles on the context's stream, edge-ring tiles on the side
-    // stream (fork after the previous work, join before the filters)
-    HIPCHK(hipEventRecord(c->fork, c->stream));
-    HIPCHK(hipStreamWaitEvent(c->side, c->fork, 0));
+    // interior kernel on the context's stream; edge tiles in the same launch
+    // or on the side stream (forked after the previous work, joined before
+    // the filters)
     StageHooks tm;
+    tm.fork = c->fork;
+    tm.join = c->join;
+    tm.same_launch = c->same_launch;
     if (timed) {
         tm.before = c->ev[5];
         tm.after = c->ev[6];
     }
-    HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream, c->side,
-                        timed ? &tm : nullptr));
-    HIPCHK(hipEventRecord(c->join, c->side));
-    HIPCHK(hipStreamWaitEvent(c->stream, c->join, 0));
+    HIPCHK(launch_stage(A, P.nk, visc, false, c->stage_variant < 0 ? 0 : c->stage_variant, c->stream, c->side, &tm));
     tick(1);
     if (c->filt_mode == 1) {
         // one fused sweep: its time is reported under "filter_x", "filter_y" gets 0
@@ -525,16 +525,16 @@ int autotune_stage_t(ns_ctx *c)
     S *S0 = static_cast<S *>(c->S[0]), *S1 = static_cast<S *>(c->S[1]);
     const StageArgs<S> A = stage_args<S>(c, 1, S0, S1, 0.0, c->pert);
     auto launch = [&](int v) -> hipError_t {
-        hipError_t e = hipEventRecord(c->fork, c->stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->fork, 0);
-        if (e == hipSuccess) e = launch_stage(A, P.nk, visc, false, v, c->stream, c->side);
-        if (e == hipSuccess) e = hipEventRecord(c->join, c->side);
-        if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->join, 0);
-        return e;
+        StageHooks h;
+        h.fork = c->fork;
+        h.join = c->join;
+        h.same_launch = c->same_launch;
+        return launch_stage(A, P.nk, visc, false, v, c->stream, c->side, &h);
     };
     int best = 0;
     double best_ms = 1e300;
     const int reps = 3;
+    c->same_launch = false;  // the variants are timed with the edge tiles on the second stream
     for (int v = 0; v < 3; ++v) {
         hipError_t e = launch(v);
         if (e != hipSuccess) {
@@ -557,6 +557,23 @@ int autotune_stage_t(ns_ctx *c)
         }
     }
     c->stage_variant = best;
+    // the chosen variant with its edge tiles in the marching launch against
+    // the second stream (a variant without a marching launch is unchanged)
+    {
+        double form_ms[2];
+        for (int f = 0; f < 2; ++f) {
+            c->same_launch = f == 1;
+            HIPCHK(launch(best));
+            HIPCHK(hipEventRecord(c->ev[0], c->stream));
+            for (int r = 0; r < reps; ++r) HIPCHK(launch(best));
+            HIPCHK(hipEventRecord(c->ev[1], c->stream));
+            HIPCHK(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            form_ms[f] = ms / reps;
+        }
+        c->same_launch = form_ms[1] < form_ms[0];
+    }
     // look-back chunk length of each filter sweep (more chunks: more waves in
     // flight; longer chunks: less warm-up work), timed on the scratch buffers
     // S1 -> S2 (x) and S2 -> S1 (y) so the state is untouched.  Lines

@@ -195,12 +195,19 @@ struct RectSet {
 
 // Optional hooks of a stage launch: events recorded on the interior
 // kernel's stream right before and after its launch, and the number of
-// output points that kernel covers; plan_only: do the host-side set-up
-// (launch plan, LDS attributes, which load the kernels) and launch nothing.
+// output points that kernel covers (the whole grid when the edge tiles run
+// in the same launch); plan_only: do the host-side set-up (launch plan, LDS
+// attributes, which load the kernels) and launch nothing.
 struct StageHooks {
     hipEvent_t before = nullptr, after = nullptr;
     long long interior_points = 0;
     bool plan_only = false;
+    // events for running the edge tiles on the second stream: without them
+    // (or without a second stream) everything runs on the first
+    hipEvent_t fork = nullptr, join = nullptr;
+    // marching stages: the small edge tiles in the marching launch instead
+    // (stage_march_edges_kernel); plan_only sets up both forms
+    bool same_launch = false;
 };
 
 struct MarchMap {

@@ -15,6 +15,7 @@
 // (the FP32 perturbation form, contraction allowed).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -510,13 +511,12 @@ struct EdgeShape {
     static constexpr int BX = SMALL ? 32 : STX, BY = SMALL ? 8 : TY;
     static constexpr int NT = SMALL ? 256 : 512;
 };
+// edge tile t of the side strips, then of the bands
 template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, bool SMALL>
-__global__ __launch_bounds__((EdgeShape<SMALL, TY>::NT)) void stage_kernel_edges(StageArgs<S> A, RectSet sides,
-                                                                                RectSet bands)
+__device__ __forceinline__ void edge_block(const StageArgs<S> &A, const RectSet &sides, const RectSet &bands, int t)
 {
     using E = EdgeShape<SMALL, TY>;
     int i0, i1, j0, j1;
-    const int t = (int)blockIdx.x;
     if (t < sides.count()) {
         sides.tile(t, i0, i1, j0, j1);
         stage_tile<S, R, NK, VISC, E::SY, RHS_ONLY, PERT, E::NT, PM_GEN, E::SX>(A, i0, i1, j0, j1);
@@ -524,6 +524,13 @@ __global__ __launch_bounds__((EdgeShape<SMALL, TY>::NT)) void stage_kernel_edges
         bands.tile(t - sides.count(), i0, i1, j0, j1);
         stage_tile<S, R, NK, VISC, E::BY, RHS_ONLY, PERT, E::NT, PM_GEN, E::BX>(A, i0, i1, j0, j1);
     }
+}
+
+template <typename S, typename R, int NK, bool VISC, int TY, bool RHS_ONLY, bool PERT, bool SMALL>
+__global__ __launch_bounds__((EdgeShape<SMALL, TY>::NT)) void stage_kernel_edges(StageArgs<S> A, RectSet sides,
+                                                                                RectSet bands)
+{
+    edge_block<S, R, NK, VISC, TY, RHS_ONLY, PERT, SMALL>(A, sides, bands, (int)blockIdx.x);
 }
 
 // Batched launch: one block per entry of a tile table spanning every case;
@@ -749,13 +756,6 @@ __device__ __forceinline__ void march_body(const StageArgs<S> A, const int i0, c
 // others PM_MIDU (the same bits, as for every point mode)
 constexpr int PM_HYBRID = 4;
 
-// this block's (strip, chunk): march_tile_of over the linear block index
-__device__ __forceinline__ void march_tile(int nstrips, int &strip, int &chunk)
-{
-    march_tile_of((int)(blockIdx.x + blockIdx.y * gridDim.x), (int)(gridDim.x * gridDim.y), nstrips, strip,
-                  chunk);
-}
-
 // FP32 blocks of 4 rows per step take 36 KB of LDS (nk = 4), so 4 fit a
 // CU: the registers must allow 4 waves per SIMD as well (the general-region
 // kernel then keeps 128 VGPRs instead of 137; nk = 5 would spill)
@@ -765,12 +765,12 @@ constexpr int march_waves_per_simd()
     return (sizeof(S) == 4 && RB == 4 && NK == 4) ? 4 : 1;
 }
 
+// marching block L of T (march_tile_of)
 template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
-__global__ __launch_bounds__(RB * 64, (march_waves_per_simd<S, RB, NK>())) void stage_march_kernel(StageArgs<S> A,
-                                                                                           MarchMap m)
+__device__ __forceinline__ void march_block(const StageArgs<S> &A, const MarchMap &m, int L, int T)
 {
     int strip, chunk;
-    march_tile(m.nstrips, strip, chunk);
+    march_tile_of(L, T, m.nstrips, strip, chunk);
     const int i0 = m.xa + strip * MTX;
     const int nout = min(MTX, m.xe - i0);
     if constexpr (PM == PM_HYBRID) {
@@ -785,6 +785,31 @@ __global__ __launch_bounds__(RB * 64, (march_waves_per_simd<S, RB, NK>())) void 
         const int y0 = m.ya + chunk * m.H;
         march_body<S, R, NK, VISC, PERT, RB, PM>(A, i0, nout, y0, min(y0 + m.H, m.yb));
     }
+}
+
+template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
+__global__ __launch_bounds__(RB * 64, (march_waves_per_simd<S, RB, NK>())) void stage_march_kernel(StageArgs<S> A,
+                                                                                           MarchMap m)
+{
+    march_block<S, R, NK, VISC, PERT, RB, PM>(A, m, (int)(blockIdx.x + blockIdx.y * gridDim.x),
+                                              (int)(gridDim.x * gridDim.y));
+}
+
+// The marching blocks and, after them in the same launch, the small edge
+// tiles around the marching region (same 256-thread blocks; the dynamic LDS
+// is the larger of the two layouts, the march's).  Blocks start in order, so
+// the edge tiles take the slots the marching blocks leave and then those
+// they free: one launch per stage, without a second stream's fork and join.
+template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM, int TY>
+__global__ __launch_bounds__(RB * 64, (march_waves_per_simd<S, RB, NK>())) void stage_march_edges_kernel(
+    StageArgs<S> A, MarchMap m, RectSet sides, RectSet bands, int nmarch)
+{
+    static_assert(RB * 64 == EdgeShape<true, TY>::NT, "marching blocks and small edge tiles share the block size");
+    const int L = (int)blockIdx.x;
+    if (L < nmarch)
+        march_block<S, R, NK, VISC, PERT, RB, PM>(A, m, L, nmarch);
+    else
+        edge_block<S, R, NK, VISC, TY, false, PERT, true>(A, sides, bands, L - nmarch);
 }
 
 template <typename R>
@@ -887,13 +912,49 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         if (inner) xa = fx0 * STX, xb = fx1 * STX, ya = fy0 * TY, yb = fy1 * TY;
     }
     hipError_t e;
+    using EL = EdgeShape<false, TY>;
+    using ES = EdgeShape<true, TY>;
+    if constexpr (MARCH && !RHS_ONLY && RB * 64 == ES::NT) {
+        // one launch: the marching blocks, then the small edge tiles
+        // (stage_march_edges_kernel); the whole grid is its work.  The edge
+        // tiles fill the slots the march leaves and frees, without the second
+        // stream's fork and join; which form is faster depends on the edge
+        // work and on the waves of marching blocks (4096^2: FP32 with its
+        // 44-point ring 4.08 -> 4.00 ms per step, FP64 with its 6-point band
+        // 6.63 -> 6.69 ms), so the autotuner picks (same_launch).
+        const bool plan = tm && tm->plan_only;
+        if (inner && tm && (tm->same_launch || plan)) {
+            const RectSet sides = RectSet::around(A.imax, A.jmax, ES::SX, ES::SY, xa, xb, ya, yb, 12);
+            const RectSet bands = RectSet::around(A.imax, A.jmax, ES::BX, ES::BY, xa, xb, ya, yb, 3);
+            size_t sm = march_smem<R>(NK, VISC, RB);
+            sm = std::max(sm, stage_smem<R>(NK, VISC, ES::SY, ES::SX));
+            sm = std::max(sm, stage_smem<R>(NK, VISC, ES::BY, ES::BX));
+            void (*comb)(StageArgs<S>, MarchMap, RectSet, RectSet, int) =
+                pm == PM_MIDU ? stage_march_edges_kernel<S, R, NK, VISC, PERT, RB, PM_MIDU, TY>
+                              : stage_march_edges_kernel<S, R, NK, VISC, PERT, RB, PM_MID, TY>;
+            if constexpr (GENERAL) {
+                if (pm == PM_HYBRID) comb = stage_march_edges_kernel<S, R, NK, VISC, PERT, RB, PM_HYBRID, TY>;
+            } else {
+                if (pm == PM_FAST) comb = stage_march_edges_kernel<S, R, NK, VISC, PERT, RB, PM_FAST, TY>;
+            }
+            if ((e = kernel_ready((const void *)comb, sm, RB * 64, s)) != hipSuccess) return e;
+            if (!plan) {
+                tm->interior_points = (long long)A.imax * A.jmax;
+                const int nmarch = m.nstrips * nchunks;
+                if (tm->before && (e = hipEventRecord(tm->before, s)) != hipSuccess) return e;
+                hipLaunchKernelGGL(comb, dim3(nmarch + sides.count() + bands.count()), dim3(RB * 64), sm, s, A, m,
+                                   sides, bands, nmarch);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+                if (tm->after && (e = hipEventRecord(tm->after, s)) != hipSuccess) return e;
+                return hipSuccess;
+            }
+        }
+    }
     // edge tiles (stage_kernel_edges): side strips, then the bands.  Small
     // tiles when the marching blocks all run in one wave: LDS freed by a
     // finished marching block is then not needed again, and a small edge
     // tile can start there beside the other one.  With more waves that LDS
     // goes to the next marching blocks, and large tiles wait for whole CUs.
-    using EL = EdgeShape<false, TY>;
-    using ES = EdgeShape<true, TY>;
     const bool small = inner && one_wave;
     const RectSet sides = small ? RectSet::around(A.imax, A.jmax, ES::SX, ES::SY, xa, xb, ya, yb, 12)
                                 : RectSet::around(A.imax, A.jmax, EL::SX, EL::SY, xa, xb, ya, yb, inner ? 12 : 0);
@@ -954,6 +1015,13 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
     }
     if (tm && tm->plan_only) return hipSuccess;
     if (!inner) return edges(s);
+    // the edge tiles on s2 start after the work queued on s so far (fork)
+    // and s continues after them (join)
+    const bool side = s2 && tm && tm->fork && tm->join;
+    if (side) {
+        if ((e = hipEventRecord(tm->fork, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s2, tm->fork, 0)) != hipSuccess) return e;
+    }
     if (tm && tm->before && (e = hipEventRecord(tm->before, s)) != hipSuccess) return e;
     if constexpr (!RHS_ONLY && MARCH) {
         hipLaunchKernelGGL(march, dim3(m.nstrips, nchunks), dim3(RB * 64), msm, s, A, m);
@@ -965,7 +1033,10 @@ static hipError_t launch_stage_t(const StageArgs<S> &A, hipStream_t s, hipStream
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (tm && tm->after && (e = hipEventRecord(tm->after, s)) != hipSuccess) return e;
     }
-    return edges(s2 ? s2 : s);
+    if (!side) return edges(s);
+    if ((e = edges(s2)) != hipSuccess) return e;
+    if ((e = hipEventRecord(tm->join, s2)) != hipSuccess) return e;
+    return hipStreamWaitEvent(s, tm->join, 0);
 }
 
 // variants (same arithmetic):
