@@ -532,48 +532,41 @@ int autotune_stage_t(ns_ctx *c)
         return launch_stage(A, P.nk, visc, false, v, c->stream, c->side, &h);
     };
     int best = 0;
+    bool best_same = false;
     double best_ms = 1e300;
     const int reps = 3;
-    c->same_launch = false;  // the variants are timed with the edge tiles on the second stream
+    // each variant with its edge tiles on the second stream and in the
+    // marching launch (a variant without a marching launch runs the same
+    // either way); tune_ms reports each variant's faster form
     for (int v = 0; v < 3; ++v) {
-        hipError_t e = launch(v);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            c->tune_ms[v] = -1.0;
-            continue;
-        }
-        HIPCHK(hipEventRecord(c->ev[0], c->stream));
-        for (int r = 0; r < reps; ++r) HIPCHK(launch(v));
-        HIPCHK(hipEventRecord(c->ev[1], c->stream));
-        HIPCHK(hipEventSynchronize(c->ev[1]));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-        c->tune_ms[v] = ms / reps;
-        // a later variant must be more than 3 % faster to replace an earlier
-        // one, so that variants within timing noise do not flip from run to run
-        if (ms / reps < best_ms * (best_ms < 1e299 ? 0.97 : 1.0)) {
-            best_ms = ms / reps;
-            best = v;
-        }
-    }
-    c->stage_variant = best;
-    // the chosen variant with its edge tiles in the marching launch against
-    // the second stream (a variant without a marching launch is unchanged)
-    {
-        double form_ms[2];
+        c->tune_ms[v] = -1.0;
         for (int f = 0; f < 2; ++f) {
             c->same_launch = f == 1;
-            HIPCHK(launch(best));
+            hipError_t e = launch(v);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
             HIPCHK(hipEventRecord(c->ev[0], c->stream));
-            for (int r = 0; r < reps; ++r) HIPCHK(launch(best));
+            for (int r = 0; r < reps; ++r) HIPCHK(launch(v));
             HIPCHK(hipEventRecord(c->ev[1], c->stream));
             HIPCHK(hipEventSynchronize(c->ev[1]));
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-            form_ms[f] = ms / reps;
+            const double t = ms / reps;
+            if (c->tune_ms[v] < 0 || t < c->tune_ms[v]) c->tune_ms[v] = t;
+            // a later candidate must be more than 3 % faster to replace an
+            // earlier one, so that candidates within timing noise do not flip
+            // from run to run
+            if (t < best_ms * (best_ms < 1e299 ? 0.97 : 1.0)) {
+                best_ms = t;
+                best = v;
+                best_same = f == 1;
+            }
         }
-        c->same_launch = form_ms[1] < form_ms[0];
     }
+    c->stage_variant = best;
+    c->same_launch = best_same;
     // look-back chunk length of each filter sweep (more chunks: more waves in
     // flight; longer chunks: less warm-up work), timed on the scratch buffers
     // S1 -> S2 (x) and S2 -> S1 (y) so the state is untouched.  Lines
