@@ -71,17 +71,19 @@ def make_case(args):
     return case, desc
 
 
-def profiled_traffic(workload: str, size: int, precision: str, variant: int):
-    """HBM bytes per stage launch of the given stage variant, measured with
-    rocprofv3 --pmc (FETCH_SIZE x2 + WRITE_SIZE, calibrated on a plain copy),
-    from the committed summary profiles/hbm_traffic_ns4096.md; None when that
+def profiled_traffic(workload: str, size: int, precision: str, variant: int, same_launch: bool = False):
+    """HBM bytes per stage launch of the given stage variant (same_launch: the
+    form whose launch also covers the edge tiles), measured with rocprofv3
+    --pmc (FETCH_SIZE x2 + WRITE_SIZE, calibrated on a plain copy), from the
+    committed summary profiles/hbm_traffic_ns4096.md; None when that
     configuration has none."""
+    key = f"v{variant}" + ("e" if same_launch else "")
     path = os.path.join(ROOT, "profiles", "hbm_traffic_ns4096.md")
     try:
         for line in open(path):
             f = line.split()
             if (len(f) == 6 and f[0] == "traffic" and f[1] == workload and int(f[2]) == size
-                    and f[3] == precision and f[4] == f"v{variant}"):
+                    and f[3] == precision and f[4] == key):
                 return int(f[5])
     except OSError:
         pass
@@ -477,9 +479,12 @@ def main():
                    "filter_tune_ms": r["filter_tune_ms"]},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(args.workload, case.imax, args.precision, variant),
-                     "kernel": "stage interior kernel (RHS + LDDRK update + buffer over the interior region; "
-                               "the edge tiles run beside it on a second stream)",
+                     "traffic": profiled_traffic(args.workload, case.imax, args.precision, variant,
+                                                 npts_int == case.npoints),
+                     "kernel": ("stage kernel (RHS + LDDRK update + buffer; marching blocks over the interior "
+                                "region and the edge tiles in one launch, whole grid)" if npts_int == case.npoints
+                                else "stage interior kernel (RHS + LDDRK update + buffer over the interior region; "
+                                "the edge tiles run beside it on a second stream)"),
                      "interior_points": npts_int,
                      "alg_bytes_per_launch": B_stage * npts_int,
                      "avg_launch_ms": round(kt["interior"], 4)},

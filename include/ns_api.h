@@ -195,7 +195,7 @@ int32_t ns_metrics_y(ns_ctx *ctx, int32_t nk, double *dU);
 /* batched cases (no reference counterpart: the reference runs one case per
    process) ------------------------------------------------------------------ */
 /* Many independent cases of any grid sizes advanced together.  Each kernel
-   class runs as one launch over a tile table that spans every case, and one
+   class runs as one launch over a work table that spans every case, and one
    captured HIP graph replays a whole step of the batch.  All cases share nk,
    precision and flags; the acoustic source must be off (amplitude 0). */
 typedef struct ns_batch ns_batch;
@@ -208,8 +208,18 @@ int32_t ns_batch_set_case(ns_batch *b, int32_t c, const double *meshx, const dou
                           const double *Ub, const double *U0);
 int32_t ns_batch_steps(ns_batch *b, int32_t iter0, int32_t nsteps);
 int32_t ns_batch_get_state(ns_batch *b, int32_t c, double *U);
-/* stage-kernel block size for the whole batch, as ns_autotune */
+/* Time the batch's stage variants and keep the fastest, as ns_autotune:
+   0 = tiles over every case (fast tiles and general tiles as two launches on
+   two streams); 1 = one marching launch over every case (4 rows per step,
+   blocks in each case's fast zone on the fast code, then the edge tiles),
+   2 = the same on plain general regions; the marching variants at the chunk
+   height that times fastest.  ms[v] = one stage launch of variant v (-1:
+   cannot run these cases).  FP64 results are bitwise the same for every
+   variant. */
 int32_t ns_batch_autotune(ns_batch *b, int32_t *variant, double *ms);
+/* Use batch stage variant 0..2 (as ns_batch_autotune) from now on; the first
+   call on a batch times every variant, which sets up their kernels. */
+int32_t ns_batch_set_variant(ns_batch *b, int32_t variant);
 int32_t ns_batch_destroy(ns_batch *b);
 
 /* device-resident state ------------------------------------------------------

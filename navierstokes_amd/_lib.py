@@ -104,6 +104,7 @@ SIGNATURES = {
     "ns_batch_steps": (C.c_int32, [_V, C.c_int32, C.c_int32]),
     "ns_batch_get_state": (C.c_int32, [_V, C.c_int32, _DP]),
     "ns_batch_autotune": (C.c_int32, [_V, _IP, _DP]),
+    "ns_batch_set_variant": (C.c_int32, [_V, C.c_int32]),
     "ns_batch_destroy": (C.c_int32, [_V]),
     "ns_put_state_dev": (C.c_int32, [_V, _V]),
     "ns_get_state_dev": (C.c_int32, [_V, _V]),

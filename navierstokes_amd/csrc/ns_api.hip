@@ -38,6 +38,12 @@ hipError_t launch_stage_batch(const StageArgs<double> *args, const int4 *ftiles,
                               int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
 hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, int nf, const int4 *gtiles, int ng,
                               int stage, int ncases, int nk, bool visc, int variant, hipStream_t s, hipStream_t s2);
+bool batch_march_case(const StageArgs<double> &A, int nk, bool visc, bool hybrid, int H, BatchMarch &p);
+bool batch_march_case(const StageArgs<float> &A, int nk, bool visc, bool hybrid, int H, BatchMarch &p);
+hipError_t launch_stage_march_batch(const StageArgs<double> *args, const BatchMarch *plans, const int4 *items,
+                                    int nitems, int stage, int ncases, int nk, bool visc, bool unit, hipStream_t s);
+hipError_t launch_stage_march_batch(const StageArgs<float> *args, const BatchMarch *plans, const int4 *items,
+                                    int nitems, int stage, int ncases, int nk, bool visc, bool unit, hipStream_t s);
 template <typename S>
 hipError_t launch_vorticity(const S *u, const S *v, const double *ub1, int imax, int jmax, int pitch, double inv_dm,
                             double inv_dn, bool xper, double *out, hipStream_t s);
@@ -1785,6 +1791,14 @@ struct ns_batch {
     double tune_ms[3] = {0, 0, 0};
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // marching variants (1: each case's fast zone marked, 2: plain general
+    // regions): per-case plans and the work-item table of the layout and
+    // chunk height built last, and the chunk height chosen per variant
+    BatchMarch *mplans = nullptr;
+    int4 *mitems = nullptr;
+    int nmi = 0, mlayout = 0, mH = -1;
+    bool munit = true;  // every case on unit spacing (else the general code with 1/h factors)
+    int march_H[3] = {0, 0, 0};
 };
 
 namespace {
@@ -1793,7 +1807,7 @@ void batch_release(ns_batch *b)
 {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (void *p : {b->sargs, b->fxargs, b->fyargs, b->eargs, (void *)b->stiles, (void *)b->gtiles, (void *)b->fxtiles,
-                    (void *)b->fytiles})
+                    (void *)b->fytiles, (void *)b->mplans, (void *)b->mitems})
         if (p) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1863,16 +1877,86 @@ int batch_prepare_t(ns_batch *b)
     return NS_OK;
 }
 
+// The marching variants' table: each case's plan (batch_march_case), then
+// work items -- whole cases to the XCD with the least work so far, largest
+// first; per XCD each case's marching blocks (chunks, strips minor), then
+// its edge tiles -- interleaved so that block L lands on XCD L % 8, padded
+// to equal lengths.  NS_ERR_STATE when a case cannot march (FP32 plane too
+// large for 32-bit offsets).
+constexpr int BATCH_TILE_NT = 1;  // tile variant: 512-thread tiles (launch_stage_batch block size 1)
+
+template <typename S>
+int batch_march_build_t(ns_batch *b, int layout, int H)
+{
+    if (b->mitems && b->mlayout == layout && b->mH == H) return NS_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));  // the old table may be in use
+    const int n = b->n;
+    std::vector<StageArgs<S>> args(n);
+    bool unit = true;  // the fast zones need the PM_HYBRID kernel, which needs unit spacing in every case
+    for (int c = 0; c < n; ++c) {
+        ns_ctx *x = b->cs[c];
+        args[c] = stage_args<S>(x, 0, static_cast<S *>(x->S[0]), static_cast<S *>(x->S[1]), 0.0, b->pert);
+        unit = unit && args[c].inv_dm == 1.0 && args[c].inv_dn == 1.0;
+    }
+    std::vector<BatchMarch> plans(n);
+    for (int c = 0; c < n; ++c)
+        if (!batch_march_case(args[c], b->nk, b->visc, layout == 1 && unit, H, plans[c]))
+            return fail(NS_ERR_STATE, "case %d: the grid is too large for the batched marching kernel", c);
+    std::vector<int> order(n);
+    for (int c = 0; c < n; ++c) order[c] = c;
+    auto pts = [&](int c) { return (long long)b->cs[c]->P.imax * b->cs[c]->P.jmax; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return pts(a) > pts(c); });
+    std::vector<std::vector<int4>> mx(NXCD), ex(NXCD);
+    long long load[NXCD] = {0};
+    for (int c : order) {
+        int x = 0;
+        for (int q = 1; q < NXCD; ++q)
+            if (load[q] < load[x]) x = q;
+        load[x] += pts(c);
+        const BatchMarch &p = plans[c];
+        for (int ch = 0; ch < p.nchunks; ++ch)
+            for (int st = 0; st < p.m.nstrips; ++st) mx[x].push_back(make_int4(c, st, ch, 0));
+        for (int t = 0; t < p.sides.count() + p.bands.count(); ++t) ex[x].push_back(make_int4(c, t, 0, 1));
+    }
+    std::vector<int4> items;
+    for (auto *lists : {&mx, &ex}) {
+        size_t len = 0;
+        for (auto &l : *lists) len = std::max(len, l.size());
+        for (size_t k = 0; k < len; ++k)
+            for (int x = 0; x < NXCD; ++x) items.push_back(k < (*lists)[x].size() ? (*lists)[x][k] : make_int4(0, 0, 0, 2));
+    }
+    if (b->mplans) (void)hipFree(b->mplans);
+    if (b->mitems) (void)hipFree(b->mitems);
+    b->mplans = nullptr;
+    b->mitems = nullptr;
+    b->mH = -1;
+    int rc;
+    if ((rc = upload_vec(b->stream, (void **)&b->mplans, plans)) ||
+        (rc = upload_vec(b->stream, (void **)&b->mitems, items)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->nmi = (int)items.size();
+    b->mlayout = layout;
+    b->mH = H;
+    b->munit = unit;
+    return NS_OK;
+}
+
 template <typename S>
 int batch_enqueue_step_t(ns_batch *b)
 {
     for (int irk = 0; irk < 6; ++irk) {
-        HIPCHK(hipEventRecord(b->fork, b->stream));
-        HIPCHK(hipStreamWaitEvent(b->side, b->fork, 0));
-        HIPCHK(launch_stage_batch(static_cast<const StageArgs<S> *>(b->sargs), b->stiles, b->nst, b->gtiles, b->ngt, irk,
-                                  b->n, b->nk, b->visc, b->variant < 0 ? 0 : b->variant, b->stream, b->side));
-        HIPCHK(hipEventRecord(b->join, b->side));
-        HIPCHK(hipStreamWaitEvent(b->stream, b->join, 0));
+        if (b->variant >= 1) {
+            HIPCHK(launch_stage_march_batch(static_cast<const StageArgs<S> *>(b->sargs), b->mplans, b->mitems, b->nmi,
+                                            irk, b->n, b->nk, b->visc, b->munit, b->stream));
+        } else {
+            HIPCHK(hipEventRecord(b->fork, b->stream));
+            HIPCHK(hipStreamWaitEvent(b->side, b->fork, 0));
+            HIPCHK(launch_stage_batch(static_cast<const StageArgs<S> *>(b->sargs), b->stiles, b->nst, b->gtiles,
+                                      b->ngt, irk, b->n, b->nk, b->visc, BATCH_TILE_NT, b->stream, b->side));
+            HIPCHK(hipEventRecord(b->join, b->side));
+            HIPCHK(hipStreamWaitEvent(b->stream, b->join, 0));
+        }
         HIPCHK(launch_filter_batch(static_cast<const FilterArgs<S> *>(b->fxargs), b->fxtiles, b->nfx, b->nk, true,
                                    b->stream));
         HIPCHK(launch_filter_batch(static_cast<const FilterArgs<S> *>(b->fyargs), b->fytiles, b->nfy, b->nk, false,
@@ -1887,36 +1971,54 @@ int batch_enqueue_step(ns_batch *b)
     return b->pert ? batch_enqueue_step_t<float>(b) : batch_enqueue_step_t<double>(b);
 }
 
-// time the stage kernel's variants on stage 2 of the batch (W dead between steps, S1 scratch)
+// Time the stage launch's variants on stage 2 of the batch (W dead between
+// steps, S1 scratch): 0 = tiles (fast tiles and general tiles on two
+// streams), 1 and 2 = the marching kernel with each case's fast zone marked
+// or on plain general regions, each at the chunk height (one chunk, 128 or
+// 64 rows) that times fastest; keeps the fastest variant.
 template <typename S>
 int batch_autotune_t(ns_batch *b)
 {
-    int best = 0;
-    double best_ms = 1e300;
-    for (int v = 0; v < 3; ++v) {
-        const StageArgs<S> *sa = static_cast<const StageArgs<S> *>(b->sargs);
-        hipError_t e =
-            launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 1, b->n, b->nk, b->visc, v, b->stream, nullptr);
-        if (e != hipSuccess) {
+    const StageArgs<S> *sa = static_cast<const StageArgs<S> *>(b->sargs);
+    // ms per launch over 3 launches after one untimed launch; -1 when the launch fails
+    auto time3 = [&](auto launch) -> double {
+        if (launch() != hipSuccess) {
             (void)hipGetLastError();
-            b->tune_ms[v] = -1.0;
-            continue;
+            return -1.0;
         }
-        HIPCHK(hipEventRecord(b->ev0, b->stream));
+        if (hipEventRecord(b->ev0, b->stream) != hipSuccess) return -1.0;
         for (int r = 0; r < 3; ++r)
-            HIPCHK(launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 1, b->n, b->nk, b->visc, v, b->stream,
-                                      nullptr));
-        HIPCHK(hipEventRecord(b->ev1, b->stream));
-        HIPCHK(hipEventSynchronize(b->ev1));
+            if (launch() != hipSuccess) return -1.0;
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        b->tune_ms[v] = ms / 3;
-        if (ms / 3 < best_ms) {
-            best_ms = ms / 3;
-            best = v;
+        if (hipEventRecord(b->ev1, b->stream) != hipSuccess || hipEventSynchronize(b->ev1) != hipSuccess ||
+            hipEventElapsedTime(&ms, b->ev0, b->ev1) != hipSuccess)
+            return -1.0;
+        return ms / 3;
+    };
+    b->tune_ms[0] = time3([&] {
+        return launch_stage_batch(sa, b->stiles, b->nst, b->gtiles, b->ngt, 1, b->n, b->nk, b->visc, BATCH_TILE_NT,
+                                  b->stream, nullptr);
+    });
+    for (int v = 1; v <= 2; ++v) {
+        b->tune_ms[v] = -1.0;
+        for (int H : {0, 128, 64}) {
+            if (batch_march_build_t<S>(b, v, H) != NS_OK) break;
+            const double t = time3([&] {
+                return launch_stage_march_batch(sa, b->mplans, b->mitems, b->nmi, 1, b->n, b->nk, b->visc, b->munit,
+                                                b->stream);
+            });
+            if (t > 0 && (b->tune_ms[v] < 0 || t < b->tune_ms[v])) {
+                b->tune_ms[v] = t;
+                b->march_H[v] = H;
+            }
         }
     }
+    int best = -1;
+    for (int v = 0; v < 3; ++v)
+        if (b->tune_ms[v] > 0 && (best < 0 || b->tune_ms[v] < b->tune_ms[best])) best = v;
+    if (best < 0) return fail(NS_ERR_HIP, "batched stage: no variant could be launched");
     b->variant = best;
+    if (best >= 1) return batch_march_build_t<S>(b, best, b->march_H[best]);
     return NS_OK;
 }
 
@@ -2040,6 +2142,25 @@ int32_t ns_batch_autotune(ns_batch *b, int32_t *variant, double *ms)
     if (variant) *variant = b->variant;
     if (ms)
         for (int v = 0; v < 3; ++v) ms[v] = b->tune_ms[v];
+    return NS_OK;
+}
+
+int32_t ns_batch_set_variant(ns_batch *b, int32_t variant)
+{
+    if (!b) return fail(NS_ERR_PARAM, "null batch");
+    if (variant < 0 || variant > 2) return fail(NS_ERR_PARAM, "batch stage variant %d (0..2)", variant);
+    int rc = batch_ready(b);  // the first call times every variant, which sets up their kernels
+    if (rc) return rc;
+    if (b->tune_ms[variant] < 0)
+        return fail(NS_ERR_STATE, "batch stage variant %d cannot run these cases", variant);
+    if (b->graph) {
+        HIPCHK(hipGraphExecDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    if (variant >= 1 && (rc = b->pert ? batch_march_build_t<float>(b, variant, b->march_H[variant])
+                                      : batch_march_build_t<double>(b, variant, b->march_H[variant])))
+        return rc;
+    b->variant = variant;
     return NS_OK;
 }
 

@@ -216,6 +216,15 @@ struct MarchMap {
     int fx0, fx1;
 };
 
+// One case of a batched marching stage (stage_march_batch_kernel): its
+// marching region and chunks, and the small edge tiles around the region
+// (the whole grid as band tiles when there is no region: nstrips = 0).
+struct BatchMarch {
+    MarchMap m;
+    RectSet sides, bands;
+    int nchunks;
+};
+
 __host__ __device__ inline int march_chunks(const MarchMap &m)
 {
     return (m.y_a > m.ya ? 1 : 0) + (m.y_b - m.y_a + m.H - 1) / m.H + (m.yb > m.y_b ? 1 : 0);
@@ -314,6 +323,50 @@ __host__ __device__ inline bool march_hybrid(int D, int imax, int jmax, int rb, 
     m.fx0 = xper ? 0 : lo;
     m.fx1 = xper ? imax : imax - lo;
     return m.y_b - m.y_a >= 4 * rb;
+}
+
+// Small edge tiles (256 threads, beside marching blocks): 16 x 16 over the
+// side strips, 32 x 8 over the bands.
+constexpr int SMALL_SX = 16, SMALL_SY = 16, SMALL_BX = 32, SMALL_BY = 8;
+
+// Edge tiles of batched marching stages (256 threads).  Without level-1
+// rows (Euler, nk = 4) the frame around each case's marching region, 6
+// columns or rows wide, takes 8 x 64 tiles on the side strips and 64 x 8 on
+// the bands: a quarter of the small tiles' count for the same points, in
+// less LDS than the march.  With level-1 rows such tiles would need more
+// LDS than the march, and the small shapes stay.
+template <bool FRAME>
+struct BatchEdgeShape {
+    static constexpr int SX = FRAME ? 8 : SMALL_SX, SY = FRAME ? 64 : SMALL_SY;
+    static constexpr int BX = FRAME ? 64 : SMALL_BX, BY = FRAME ? 8 : SMALL_BY;
+};
+
+// The plan of one case of a batched marching stage: the general region with
+// the fast zone marked (march_hybrid) when `hybrid` allows it and the case
+// has one, else the plain general region (march_region); chunks of H rows
+// between the edge chunks (H <= 0: one chunk); the edge tiles around the
+// region (BatchEdgeShape<frame>), or the whole grid as band tiles when the
+// case is too small to march (nstrips = 0, nchunks = 0).
+template <bool FRAME>
+inline void batch_march_plan(int imax, int jmax, int D, bool xper, bool hybrid, int rb, int H, BatchMarch &p)
+{
+    using E = BatchEdgeShape<FRAME>;
+    MarchMap m{};
+    bool ok = hybrid && march_hybrid(D, imax, jmax, rb, m, xper);
+    if (!ok) ok = march_region(false, D, imax, jmax, rb, m, xper);
+    int xa = 0, xb = 0, ya = 0, yb = 0;
+    if (ok) {
+        const int rows = m.y_b - m.y_a;
+        const int h = H > 0 && H < rows ? H : rows;
+        m.H = (h + rb - 1) / rb * rb;
+        xa = m.xa, xb = m.xe, ya = m.ya, yb = m.yb;
+    } else {
+        m = MarchMap{};
+    }
+    p.m = m;
+    p.nchunks = ok ? march_chunks(m) : 0;
+    p.sides = RectSet::around(imax, jmax, E::SX, E::SY, xa, xb, ya, yb, ok ? 12 : 0);
+    p.bands = RectSet::around(imax, jmax, E::BX, E::BY, xa, xb, ya, yb, ok ? 3 : 15);
 }
 
 }  // namespace nsamd

@@ -507,8 +507,8 @@ __global__ __launch_bounds__(NT) void stage_kernel_fast(StageArgs<S> A, int bx0,
 //     fits on a CU beside a marching block.
 template <bool SMALL, int TY>
 struct EdgeShape {
-    static constexpr int SX = 16, SY = SMALL ? 16 : 4 * TY;
-    static constexpr int BX = SMALL ? 32 : STX, BY = SMALL ? 8 : TY;
+    static constexpr int SX = SMALL ? SMALL_SX : 16, SY = SMALL ? SMALL_SY : 4 * TY;
+    static constexpr int BX = SMALL ? SMALL_BX : STX, BY = SMALL ? SMALL_BY : TY;
     static constexpr int NT = SMALL ? 256 : 512;
 };
 // edge tile t of the side strips, then of the bands
@@ -765,12 +765,10 @@ constexpr int march_waves_per_simd()
     return (sizeof(S) == 4 && RB == 4 && NK == 4) ? 4 : 1;
 }
 
-// marching block L of T (march_tile_of)
+// the marching block of (strip, chunk)
 template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
-__device__ __forceinline__ void march_block(const StageArgs<S> &A, const MarchMap &m, int L, int T)
+__device__ __forceinline__ void march_block_at(const StageArgs<S> &A, const MarchMap &m, int strip, int chunk)
 {
-    int strip, chunk;
-    march_tile_of(L, T, m.nstrips, strip, chunk);
     const int i0 = m.xa + strip * MTX;
     const int nout = min(MTX, m.xe - i0);
     if constexpr (PM == PM_HYBRID) {
@@ -785,6 +783,15 @@ __device__ __forceinline__ void march_block(const StageArgs<S> &A, const MarchMa
         const int y0 = m.ya + chunk * m.H;
         march_body<S, R, NK, VISC, PERT, RB, PM>(A, i0, nout, y0, min(y0 + m.H, m.yb));
     }
+}
+
+// marching block L of T (march_tile_of)
+template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
+__device__ __forceinline__ void march_block(const StageArgs<S> &A, const MarchMap &m, int L, int T)
+{
+    int strip, chunk;
+    march_tile_of(L, T, m.nstrips, strip, chunk);
+    march_block_at<S, R, NK, VISC, PERT, RB, PM>(A, m, strip, chunk);
 }
 
 template <typename S, typename R, int NK, bool VISC, bool PERT, int RB, int PM>
@@ -1150,6 +1157,129 @@ hipError_t launch_stage_batch(const StageArgs<float> *args, const int4 *ftiles, 
 {
     return launch_stage_batch_p<float, true>(args, ftiles, nf, gtiles, ng, stage, ncases, nk, visc, variant, s,
                                                  s2);
+}
+#endif
+
+// ---------------------------------------------------------------------------
+// Batched marching stage: one launch per stage over every case of a batch.
+// Each block takes one work item (case, a, b, kind) of a table the host
+// builds (batch_march_case gives each case's plan): kind 0 marches strip a,
+// chunk b of the case's region (as stage_march_kernel, 4 rows per step),
+// kind 1 computes edge tile a of the case's frame (BatchEdgeShape), kind 2
+// is padding.
+// Blocks go to the XCDs in turn (block L to XCD L % 8), and the table gives
+// each XCD whole cases, so a case's neighbouring strips and consecutive
+// chunks share one L2; the marching items come first, and the edge tiles
+// after them take the slots the march leaves.  PM_HYBRID: blocks inside a
+// case's fast zone take the fast code, the others PM_MIDU (a plain plan
+// marks no fast zone); PM_MID: the general code with 1/h factors everywhere
+// (batches with a case of non-unit spacing).
+// ---------------------------------------------------------------------------
+constexpr int BRB = 4;  // rows per marching step in batches
+
+template <int NK, bool VISC>
+constexpr bool batch_frame_tiles()
+{
+    return L1<NK, VISC>::N == 0;
+}
+
+template <typename S, typename R, int NK, bool VISC, bool PERT, int PM>
+__global__ __launch_bounds__(BRB * 64, (march_waves_per_simd<S, BRB, NK>())) void stage_march_batch_kernel(
+    const StageArgs<S> *args, const BatchMarch *plans, const int4 *items, int stage, int ncases)
+{
+    using E = BatchEdgeShape<batch_frame_tiles<NK, VISC>()>;
+    const int4 t = items[blockIdx.x];
+    if (t.w == 2) return;
+    // copies, as in stage_kernel_batch
+    const StageArgs<S> A = args[(size_t)stage * ncases + t.x];
+    if (t.w == 0) {
+        const MarchMap m = plans[t.x].m;
+        march_block_at<S, R, NK, VISC, PERT, BRB, PM>(A, m, t.y, t.z);
+    } else {
+        const RectSet sides = plans[t.x].sides, bands = plans[t.x].bands;
+        int i0, i1, j0, j1;
+        if (t.y < sides.count()) {
+            sides.tile(t.y, i0, i1, j0, j1);
+            stage_tile<S, R, NK, VISC, E::SY, false, PERT, BRB * 64, PM_GEN, E::SX>(A, i0, i1, j0, j1);
+        } else {
+            bands.tile(t.y - sides.count(), i0, i1, j0, j1);
+            stage_tile<S, R, NK, VISC, E::BY, false, PERT, BRB * 64, PM_GEN, E::BX>(A, i0, i1, j0, j1);
+        }
+    }
+}
+
+// The plan of one case (batch_march_plan, 4 rows per step): hybrid allows
+// its fast zone to be marked (unit spacing and fast-zone conditions are
+// checked here).  False when an FP32 plane is too large for 32-bit offsets.
+template <typename S>
+static bool batch_march_case_t(const StageArgs<S> &A, int nk, bool visc, bool hybrid, int H, BatchMarch &p)
+{
+    if (sizeof(S) == 4 && (size_t)A.pitch * (size_t)A.jmax * sizeof(S) >= ((size_t)1 << 32)) return false;
+    const bool unit = A.inv_dm == 1.0 && A.inv_dn == 1.0;
+    hybrid = hybrid && unit && A.fast_ok;
+    if (nk == 4 && !visc)
+        batch_march_plan<batch_frame_tiles<4, false>()>(A.imax, A.jmax, A.D, A.xper, hybrid, BRB, H, p);
+    else
+        batch_march_plan<false>(A.imax, A.jmax, A.D, A.xper, hybrid, BRB, H, p);
+    return true;
+}
+
+template <typename S, bool PERT, int NK, bool VISC, int PM>
+static hipError_t launch_stage_march_batch_t(const StageArgs<S> *args, const BatchMarch *plans, const int4 *items,
+                                             int nitems, int stage, int ncases, hipStream_t s)
+{
+    using R = S;
+    using E = BatchEdgeShape<batch_frame_tiles<NK, VISC>()>;
+    size_t sm = march_smem<R>(NK, VISC, BRB);
+    sm = std::max(sm, stage_smem<R>(NK, VISC, E::SY, E::SX));
+    sm = std::max(sm, stage_smem<R>(NK, VISC, E::BY, E::BX));
+    auto kern = stage_march_batch_kernel<S, R, NK, VISC, PERT, PM>;
+    hipError_t e = kernel_ready((const void *)kern, sm, BRB * 64, s);
+    if (e != hipSuccess || nitems <= 0) return e;
+    hipLaunchKernelGGL(kern, dim3(nitems), dim3(BRB * 64), sm, s, args, plans, items, stage, ncases);
+    return hipGetLastError();
+}
+
+template <typename S, bool PERT, int NK, bool VISC>
+static hipError_t launch_stage_march_batch_v(const StageArgs<S> *args, const BatchMarch *plans, const int4 *items,
+                                             int nitems, int stage, int ncases, bool unit, hipStream_t s)
+{
+    return unit ? launch_stage_march_batch_t<S, PERT, NK, VISC, PM_HYBRID>(args, plans, items, nitems, stage, ncases, s)
+                : launch_stage_march_batch_t<S, PERT, NK, VISC, PM_MID>(args, plans, items, nitems, stage, ncases, s);
+}
+
+template <typename S, bool PERT>
+static hipError_t launch_stage_march_batch_p(const StageArgs<S> *args, const BatchMarch *plans, const int4 *items,
+                                             int nitems, int stage, int ncases, int nk, bool visc, bool unit,
+                                             hipStream_t s)
+{
+    if (nk == 4)
+        return visc ? launch_stage_march_batch_v<S, PERT, 4, true>(args, plans, items, nitems, stage, ncases, unit, s)
+                    : launch_stage_march_batch_v<S, PERT, 4, false>(args, plans, items, nitems, stage, ncases, unit, s);
+    return visc ? launch_stage_march_batch_v<S, PERT, 5, true>(args, plans, items, nitems, stage, ncases, unit, s)
+                : launch_stage_march_batch_v<S, PERT, 5, false>(args, plans, items, nitems, stage, ncases, unit, s);
+}
+
+#if defined(NS_KERNELS_F64)
+bool batch_march_case(const StageArgs<double> &A, int nk, bool visc, bool hybrid, int H, BatchMarch &p)
+{
+    return batch_march_case_t(A, nk, visc, hybrid, H, p);
+}
+hipError_t launch_stage_march_batch(const StageArgs<double> *args, const BatchMarch *plans, const int4 *items,
+                                    int nitems, int stage, int ncases, int nk, bool visc, bool unit, hipStream_t s)
+{
+    return launch_stage_march_batch_p<double, false>(args, plans, items, nitems, stage, ncases, nk, visc, unit, s);
+}
+#endif
+#if defined(NS_KERNELS_F32)
+bool batch_march_case(const StageArgs<float> &A, int nk, bool visc, bool hybrid, int H, BatchMarch &p)
+{
+    return batch_march_case_t(A, nk, visc, hybrid, H, p);
+}
+hipError_t launch_stage_march_batch(const StageArgs<float> *args, const BatchMarch *plans, const int4 *items,
+                                    int nitems, int stage, int ncases, int nk, bool visc, bool unit, hipStream_t s)
+{
+    return launch_stage_march_batch_p<float, true>(args, plans, items, nitems, stage, ncases, nk, visc, unit, s);
 }
 #endif
 

@@ -375,7 +375,14 @@ class BatchSolver:
         return U
 
     def autotune(self):
+        """Time the stage variants (0: tiles, 1: marching with fast zones,
+        2: marching on plain general regions) and keep the fastest; returns
+        (choice, ms[3]), -1 for a variant that cannot run these cases."""
         v = C.c_int32()
         ms = np.zeros(3)
         _lib.check(self.lib.ns_batch_autotune(self.b, C.byref(v), _lib.dptr(ms)))
         return v.value, ms
+
+    def set_variant(self, variant: int):
+        """Use stage variant 0..2 (see autotune)."""
+        _lib.check(self.lib.ns_batch_set_variant(self.b, int(variant)))

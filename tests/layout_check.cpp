@@ -8,8 +8,9 @@
 // every block that takes the fast per-point code must have its outputs'
 // 3-point level-1 halo inside the fast zone; and the XCD numbering of the
 // marching blocks (march_tile_of) must be a bijection onto the tiles with
-// each XCD's tiles consecutive.  Prints "ok <layouts>" or the first
-// failure, and exits non-zero on failure.
+// each XCD's tiles consecutive; and a batched case's marching blocks and
+// small edge tiles (batch_march_plan) must cover its grid once.  Prints "ok
+// <layouts>" or the first failure, and exits non-zero on failure.
 #include <cstdio>
 #include <vector>
 
@@ -139,6 +140,57 @@ static int check_tiles(int nstrips, int nchunks)
     return 0;
 }
 
+// batched marching stage (batch_march_plan): one case's marching blocks
+// plus its small edge tiles cover the grid once, windows stay inside the
+// grid, and on a plain plan (no fast zone marked) the chunk rows of the
+// general-code path (ya + c H, clipped at yb) are those of march_chunk_rows
+template <bool FRAME>
+static int check_batch(int imax, int jmax, int D, bool hybrid, int H, long &layouts)
+{
+    const int rb = 4;
+    using E = BatchEdgeShape<FRAME>;
+    BatchMarch p{};
+    batch_march_plan<FRAME>(imax, jmax, D, false, hybrid, rb, H, p);
+    const MarchMap &m = p.m;
+    ++layouts;
+    std::vector<int> cnt((size_t)imax * jmax, 0);
+    const bool plain = m.y_a == m.ya && m.y_b == m.yb;
+    if (p.nchunks != (m.nstrips ? march_chunks(m) : 0)) return fail("batch chunks", imax, jmax, D, rb, 10 + hybrid, H);
+    for (int c = 0; c < p.nchunks; ++c)
+        for (int s = 0; s < m.nstrips; ++s) {
+            int y0, y1;
+            march_chunk_rows(m, c, y0, y1);
+            if (plain) {
+                const int z0 = m.ya + c * m.H, z1 = z0 + m.H < m.yb ? z0 + m.H : m.yb;
+                if (z0 != y0 || z1 != y1) return fail("batch plain chunk rows", imax, jmax, D, rb, 10 + hybrid, H);
+            } else if (!hybrid) {
+                return fail("batch plan marks a fast zone", imax, jmax, D, rb, 10 + hybrid, H);
+            }
+            const int i0 = m.xa + 58 * s, ie = i0 + 58 < m.xe ? i0 + 58 : m.xe;
+            if (y1 <= y0 || (y1 - y0) % rb || ie <= i0 || y0 - 6 < 0 || y1 + 5 > jmax - 1 || i0 - 6 < 0 ||
+                ie + 5 > imax - 1)
+                return fail("batch marching window", imax, jmax, D, rb, 10 + hybrid, H);
+            for (int j = y0; j < y1; ++j)
+                for (int i = i0; i < ie; ++i) ++cnt[(size_t)j * imax + i];
+        }
+    for (int t = 0; t < p.sides.count() + p.bands.count(); ++t) {
+        int i0, i1, j0, j1;
+        if (t < p.sides.count())
+            p.sides.tile(t, i0, i1, j0, j1);
+        else
+            p.bands.tile(t - p.sides.count(), i0, i1, j0, j1);
+        const bool side = t < p.sides.count();
+        if (i1 - i0 > (side ? E::SX : E::BX) || j1 - j0 > (side ? E::SY : E::BY) || i0 < 0 || j0 < 0 || i1 > imax ||
+            j1 > jmax)
+            return fail("batch edge tile", imax, jmax, D, rb, 10 + hybrid, H);
+        for (int j = j0; j < j1; ++j)
+            for (int i = i0; i < i1; ++i) ++cnt[(size_t)j * imax + i];
+    }
+    for (int v : cnt)
+        if (v != 1) return fail("batch coverage", imax, jmax, D, rb, 10 + hybrid, H);
+    return 0;
+}
+
 int main()
 {
     long layouts = 0;
@@ -161,6 +213,15 @@ int main()
         for (int jmax : sizes)
             for (int ty : {8, 16})
                 if (check_whole(imax, jmax, ty)) return 1;
+    const int bsizes[] = {40, 64, 69, 70, 96, 128, 160, 192, 256, 384, 512, 201};
+    for (int imax : bsizes)
+        for (int jmax : bsizes)
+            for (int D : {4, 8, 12, 20, 30})
+                for (bool hybrid : {false, true})
+                    for (int H : {0, 4, 64, 128, 130})
+                        if (check_batch<false>(imax, jmax, D, hybrid, H, layouts) ||
+                            check_batch<true>(imax, jmax, D, hybrid, H, layouts))
+                            return 1;
     std::printf("ok %ld\n", layouts);
     return 0;
 }
