@@ -1776,17 +1776,6 @@ __device__ __forceinline__ void filter_y_tile(const FilterArgs<R> &F, int colblo
     }
 }
 
-// y sweep blocks: FP32 rows are 256 bytes per 64-column wave, so a block
-// takes 4 waves on adjacent column blocks, which walk down the same rows
-// together (1 KB of each row at a time: FP32 y sweep 0.128 -> 0.116 ms at
-// 4096^2); FP64 rows are 512 bytes per wave already, and 4-wave blocks
-// measured slower there
-template <typename R>
-constexpr int filter_y_waves()
-{
-    return sizeof(R) == 4 ? 4 : 1;
-}
-
 template <typename R>
 __global__ __launch_bounds__(64 * filter_y_waves<R>()) void filter_y_kernel(FilterArgs<R> F)
 {
@@ -1795,7 +1784,10 @@ __global__ __launch_bounds__(64 * filter_y_waves<R>()) void filter_y_kernel(Filt
     if (cb * 64 < F.m) filter_y_tile<R>(F, cb, blockIdx.y, blockIdx.z);
 }
 
-// batched: tiles[blockIdx.x] = (case, column block, chunk), blockIdx.y = variable
+// batched: tiles[blockIdx.x] = (case, column block, chunk), blockIdx.y =
+// variable; one wave per block (4-wave blocks over adjacent column blocks,
+// as the single-grid FP32 sweep takes, measured no faster here: 204.9
+// against 203.5 us per FP32 launch on the 256-case sweep)
 template <typename R>
 __global__ __launch_bounds__(64) void filter_y_kernel_batch(const FilterArgs<R> *args, const int4 *tiles)
 {
@@ -2010,13 +2002,14 @@ template <typename S>
 static hipError_t launch_filter_batch_t(const FilterArgs<S> *args, const int4 *tiles, int ntiles, int nk, bool xdir,
                                         hipStream_t s)
 {
+    const int nt = 64;
     hipError_t e = kernel_ready(
-        xdir ? (const void *)filter_x_kernel_batch<S> : (const void *)filter_y_kernel_batch<S>, 0, 64, s);
+        xdir ? (const void *)filter_x_kernel_batch<S> : (const void *)filter_y_kernel_batch<S>, 0, nt, s);
     if (e != hipSuccess) return e;
     if (xdir)
-        hipLaunchKernelGGL(filter_x_kernel_batch<S>, dim3(ntiles, nk), dim3(64), 0, s, args, tiles);
+        hipLaunchKernelGGL(filter_x_kernel_batch<S>, dim3(ntiles, nk), dim3(nt), 0, s, args, tiles);
     else
-        hipLaunchKernelGGL(filter_y_kernel_batch<S>, dim3(ntiles, nk), dim3(64), 0, s, args, tiles);
+        hipLaunchKernelGGL(filter_y_kernel_batch<S>, dim3(ntiles, nk), dim3(nt), 0, s, args, tiles);
     return hipGetLastError();
 }
 
@@ -2087,12 +2080,14 @@ __global__ void edge_i_kernel(EdgeArgs<S> E)
     edge_i_point<S>(E, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
-// batched: blockIdx.x = case, threads stride over the rows
+// batched: blockIdx.x = case, blockIdx.y = column, blockIdx.z = block of
+// 256 rows (rows past the case's last are skipped; larger cases loop)
 template <typename S>
 __global__ void edge_i_kernel_batch(const EdgeArgs<S> *args)
 {
     const EdgeArgs<S> &E = args[blockIdx.x];
-    for (int j = threadIdx.x; j < E.jmax; j += blockDim.x) edge_i_point<S>(E, j, blockIdx.y);
+    for (int j = (int)(blockIdx.z * blockDim.x + threadIdx.x); j < E.jmax; j += (int)(gridDim.z * blockDim.x))
+        edge_i_point<S>(E, j, blockIdx.y);
 }
 
 template <typename S>
@@ -2110,7 +2105,7 @@ static hipError_t launch_edge_batch_t(const EdgeArgs<S> *args, int ncases, hipSt
 {
     hipError_t e = kernel_ready((const void *)edge_i_kernel_batch<S>, 0, 256, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(edge_i_kernel_batch<S>, dim3(ncases, 2), dim3(256), 0, s, args);
+    hipLaunchKernelGGL(edge_i_kernel_batch<S>, dim3(ncases, 2, 2), dim3(256), 0, s, args);
     return hipGetLastError();
 }
 
