@@ -145,12 +145,12 @@ static int check_tiles(int nstrips, int nchunks)
 // grid, and on a plain plan (no fast zone marked) the chunk rows of the
 // general-code path (ya + c H, clipped at yb) are those of march_chunk_rows
 template <bool FRAME>
-static int check_batch(int imax, int jmax, int D, bool hybrid, int H, long &layouts)
+static int check_batch(int imax, int jmax, int D, bool hybrid, int H, long &layouts, bool xper = false)
 {
     const int rb = 4;
     using E = BatchEdgeShape<FRAME>;
     BatchMarch p{};
-    batch_march_plan<FRAME>(imax, jmax, D, false, hybrid, rb, H, p);
+    batch_march_plan<FRAME>(imax, jmax, D, xper, hybrid, rb, H, p);
     const MarchMap &m = p.m;
     ++layouts;
     std::vector<int> cnt((size_t)imax * jmax, 0);
@@ -167,8 +167,9 @@ static int check_batch(int imax, int jmax, int D, bool hybrid, int H, long &layo
                 return fail("batch plan marks a fast zone", imax, jmax, D, rb, 10 + hybrid, H);
             }
             const int i0 = m.xa + 58 * s, ie = i0 + 58 < m.xe ? i0 + 58 : m.xe;
-            if (y1 <= y0 || (y1 - y0) % rb || ie <= i0 || y0 - 6 < 0 || y1 + 5 > jmax - 1 || i0 - 6 < 0 ||
-                ie + 5 > imax - 1)
+            if (y1 <= y0 || (y1 - y0) % rb || ie <= i0 || y0 - 6 < 0 || y1 + 5 > jmax - 1 ||
+                (!xper && (i0 - 6 < 0 || ie + 5 > imax - 1)) ||
+                (xper && (wrap_col(i0 - 6, imax) < 0 || wrap_col(i0 + 63, imax) >= imax)))
                 return fail("batch marching window", imax, jmax, D, rb, 10 + hybrid, H);
             for (int j = y0; j < y1; ++j)
                 for (int i = i0; i < ie; ++i) ++cnt[(size_t)j * imax + i];
@@ -219,9 +220,10 @@ int main()
             for (int D : {4, 8, 12, 20, 30})
                 for (bool hybrid : {false, true})
                     for (int H : {0, 4, 64, 128, 130})
-                        if (check_batch<false>(imax, jmax, D, hybrid, H, layouts) ||
-                            check_batch<true>(imax, jmax, D, hybrid, H, layouts))
-                            return 1;
+                        for (bool xper : {false, true})
+                            if ((!xper || imax >= 64) && (check_batch<false>(imax, jmax, D, hybrid, H, layouts, xper) ||
+                                                          check_batch<true>(imax, jmax, D, hybrid, H, layouts, xper)))
+                                return 1;
     std::printf("ok %ld\n", layouts);
     return 0;
 }

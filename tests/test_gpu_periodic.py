@@ -111,10 +111,15 @@ def test_periodic_fp32_perturbation():
     s32.close()
 
 
-def test_periodic_batch_equals_single():
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_periodic_batch_equals_single(variant):
+    """Every batch stage variant (tiles; marching, whose strips wrap around
+    the period) gives the single contexts' FP64 bits."""
     from navierstokes_amd.solver import BatchSolver, Solver
     cases = [_case(0), _case(2)]   # both nk = 4
+    print(f"[case] periodic batch, variant {variant}", flush=True)
     b = BatchSolver(cases)
+    b.set_variant(variant)
     b.run(0, 3)
     for i, c in enumerate(cases):
         s = Solver(c)
