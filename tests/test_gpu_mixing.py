@@ -1,7 +1,7 @@
 """Mixing layer on the tanh-stretched grid (metrics everywhere): one FP64
 step at the full 2048 x 1024 size against the oracle, the device row
 average, and the FP32 perturbation form against FP64 through the momentum
-thickness the mixing-layer runs are judged by."""
+thickness, whose growth rate is the mixing layer's figure of merit."""
 import numpy as np
 import pytest
 
