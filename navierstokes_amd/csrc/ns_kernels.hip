@@ -480,10 +480,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<S> &A, int i0, int i1
         };
         auto lv = [=](int l) { return sL[l * LY * LX + (gy - ly0) * LX + (gx - lx0)]; };
         R qc[NK], uc[NK], f[NK];
+        // the update's loads (W, buffer sigmas, base state) issued ahead of
+        // the RHS: issued in the update, each would wait behind the stores
+        // of the planes before it, which may alias it
+        StorePre<S, NK> spre;
+        if constexpr (!RHS_ONLY) store_prefetch<S, NK, PERT, PM>(A, gx, gy, spre);
         rhs_point<S, R, NK, VISC, PERT, PM>(A, gx, gy, PM == PM_FAST ? R(1) : R(A.metx[gx]),
                                             PM == PM_FAST ? R(1) : R(A.mety[gy]), ux_at(gy), uy_at(gx), lxa, lya,
                                             lv, qc, uc, f);
-        stage_store<S, R, NK, RHS_ONLY, PERT, PM>(A, gx, gy, qc, uc, f);
+        stage_store<S, R, NK, RHS_ONLY, PERT, PM>(A, gx, gy, qc, uc, f, RHS_ONLY ? nullptr : &spre);
     }
 }
 
