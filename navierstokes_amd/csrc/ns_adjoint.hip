@@ -242,27 +242,38 @@ __global__ __launch_bounds__(256) void adj_point_kernel(AdjPointArgs<S> P)
     if (ybuf) bs = bs * R(A.sigy[j]);
     if (xbuf) bs = bs * R(A.sigx[i]);
     const bool interior = P.rhs_only || (i >= 1 && i <= imax - 2 && j >= 1 && j <= jmax - 2);
-    R lamp[4], Lf[4];
+    // every load of the point is issued before its first store: a load
+    // after a store to another plane would wait for it (the planes may alias
+    // as far as the compiler knows), and the planes' round trips would run
+    // one after another
+    R lamin[4], lamw[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
+        lamin[k] = P.lam[k][g];
+        lamw[k] = (!P.rhs_only && interior) ? P.lamW[k][g] : R(0);
+    }
+    R lamp[4], Lf[4], lwo[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        lwo[k] = R(0);
         if (P.rhs_only) {  // R'^T alone, at every point (one-sided rows at the edges)
             lamp[k] = R(0);
-            Lf[k] = P.lam[k][g];
+            Lf[k] = lamin[k];
             continue;
         }
-        lamp[k] = bs * P.lam[k][g];
+        lamp[k] = bs * lamin[k];
         if (interior) {
-            const R wt = P.lamW[k][g] + R(A.beta) * lamp[k];
+            const R wt = lamw[k] + R(A.beta) * lamp[k];
             Lf[k] = R(A.dt) * wt;
-            P.lamWo[k][g] = A.first ? R(0) : R(A.alpha) * wt;
+            lwo[k] = A.first ? R(0) : R(A.alpha) * wt;
         } else {
             Lf[k] = R(0);
-            P.lamWo[k][g] = R(0);
         }
     }
     if (!interior) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
+            P.lamWo[k][g] = lwo[k];
             P.acc[k][g] = lamp[k];
             P.gx[k][g] = R(0);
             P.gy[k][g] = R(0);
@@ -308,6 +319,7 @@ __global__ __launch_bounds__(256) void adj_point_kernel(AdjPointArgs<S> P)
     gy[3] = -Lf[2] * ir - Lf[3] * v;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
+        if (!P.rhs_only) P.lamWo[k][g] = lwo[k];
         P.acc[k][g] = lamp[k] + acc[k];
         P.gx[k][g] = mx * gx[k];
         P.gy[k][g] = my * gy[k];
@@ -360,13 +372,22 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(AdjGatherArgs<S> G)
     if (i >= G.imax || j >= G.jmax) return;
     const int pitch = G.pitch;
     const size_t g = (size_t)j * pitch + i;
-    for (int k = 0; k < G.nk; ++k) {
+    // all planes' loads before the first store (see adj_point_kernel)
+    R o[NKMAX];
+#pragma unroll
+    for (int k = 0; k < NKMAX; ++k) {
+        if (k >= G.nk) break;
         const R *rowx = G.gx[k] + (size_t)j * pitch;
         const R *coly = G.gy[k] + i;
         const R dx = drp_transpose<R>([rowx](int q) { return rowx[q]; }, i, G.imax, R(G.inv_dm));
         const R dy = drp_transpose<R>([coly, pitch](int q) { return coly[(size_t)q * pitch]; }, j, G.jmax,
                                       R(G.inv_dn));
-        G.out[k][g] = G.acc[k][g] + dx + dy;
+        o[k] = G.acc[k][g] + dx + dy;
+    }
+#pragma unroll
+    for (int k = 0; k < NKMAX; ++k) {
+        if (k >= G.nk) break;
+        G.out[k][g] = o[k];
     }
 }
 
