@@ -1008,7 +1008,10 @@ int32_t ns_create(const ns_params *p, ns_ctx **out)
     // slowed its stage kernel by 2-10 %, so they keep the multiple of 64.
     c->pitch = (P.imax + 63) / 64 * 64;
     if (!c->pert && ((size_t)c->pitch * sizeof(double)) % 4096 == 0) c->pitch += 64;
-    c->plane = (size_t)c->pitch * P.jmax;
+    // plane stride: 512 B more than the plane, so that the nk planes' same
+    // point starts in different HBM channels (FP32 4096^2 step 3.957 ->
+    // 3.928 ms; FP64 unchanged; 2 KB and 8 KB measured no better)
+    c->plane = (size_t)c->pitch * P.jmax + 512 / c->esz;
     auto bail = [&](int code) {
         release(c);
         delete c;
