@@ -210,15 +210,15 @@ struct StageHooks {
     bool same_launch = false;
 };
 
-// y filter sweep blocks: FP32 rows are 256 bytes per 64-column wave, so a
-// block takes 4 waves on adjacent column blocks, which walk down the same
-// rows together (1 KB of each row at a time: FP32 y sweep 0.128 -> 0.116 ms
-// at 4096^2); FP64 rows are 512 bytes per wave already, and 4-wave blocks
-// measured slower there
+// y filter sweep blocks: 4 waves on adjacent column blocks, which walk down
+// the same rows together (1 KB of each FP32 row, 2 KB of each FP64 row at a
+// time: FP32 y sweep 0.128 -> 0.116 ms at 4096^2; FP64 step 6.444 -> 6.401
+// ms once its rows were padded, where before the pad 4-wave blocks had
+// measured slower in FP64)
 template <typename R>
 constexpr int filter_y_waves()
 {
-    return sizeof(R) == 4 ? 4 : 1;
+    return 4;
 }
 
 struct MarchMap {
