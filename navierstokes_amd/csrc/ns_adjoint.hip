@@ -28,14 +28,13 @@ namespace nsamd {
 // column i=imax, the top row j=jmax and the bottom row j=1 (cc.f90:23-59);
 // components 0-2 (and 4 at the outflow) extrapolate as 2 u(2) - u(3), the
 // rest are constants.  The transpose runs the four sets in reverse order
-// (one launch each, phase 0 = bottom ... 3 = inflow): for an extrapolated
+// (phase 0 = bottom ... 3 = inflow): for an extrapolated
 // value, lam(2) += 2 lam(1), lam(3) -= lam(1); every overwritten value's
 // adjoint is then zero.
 // ---------------------------------------------------------------------------
 template <typename S>
-__global__ void adj_edge_kernel(AdjEdgeArgs<S> E, int phase)
+__device__ __forceinline__ void adj_edge_item(const AdjEdgeArgs<S> &E, int phase, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int imax = E.imax, jmax = E.jmax, pitch = E.pitch;
     auto at = [&](int k, int i, int j) -> S & { return E.lam[k][(size_t)j * pitch + i]; };
     auto extrap = [&](int k, int i0, int j0, int i1, int j1, int i2, int j2) {
@@ -63,6 +62,18 @@ __global__ void adj_edge_kernel(AdjEdgeArgs<S> E, int phase)
             else at(k, 0, t) = S(0);
         }
     }
+}
+
+// Sets first..last in order, thread t taking position t of each.  A row
+// set at position t reads and writes only column t, a column set only row
+// t, so one thread can run both row sets at its column (one launch), then
+// both column sets at its row (a second launch: they read corner values
+// the row sets changed).
+template <typename S>
+__global__ void adj_edge_kernel(AdjEdgeArgs<S> E, int first, int last)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int phase = first; phase <= last; ++phase) adj_edge_item<S>(E, phase, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -473,9 +484,9 @@ static hipError_t launch_adj_filter_t(const AdjFilterArgs<S> &F, bool ydir, hipS
 template <typename S>
 static hipError_t launch_adj_edge_t(const AdjEdgeArgs<S> &E, hipStream_t s)
 {
-    for (int phase = 0; phase < 4; ++phase) {
-        const int n = phase < 2 ? E.imax : E.jmax;
-        hipLaunchKernelGGL(adj_edge_kernel<S>, dim3((n + 255) / 256), dim3(256), 0, s, E, phase);
+    for (int first = 0; first < 4; first += 2) {
+        const int n = first < 2 ? E.imax : E.jmax;
+        hipLaunchKernelGGL(adj_edge_kernel<S>, dim3((n + 255) / 256), dim3(256), 0, s, E, first, first + 1);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
