@@ -104,17 +104,121 @@ struct LineRef {
     __host__ __device__ __forceinline__ S &operator()(int q) const { return p[(size_t)q * st]; }
 };
 
-template <typename S, bool YDIR>
-__host__ __device__ __forceinline__ void adj_filter_line(const LineRef<const S> &In, const LineRef<S> &Out, int n,
-                                                         S sigma, int a, int b, int W)
+template <typename S>
+__host__ __device__ __forceinline__ S line_rd(const LineRef<const S> &r, int q)
+{
+    return r(q);
+}
+template <typename S>
+__host__ __device__ __forceinline__ void line_wr(const LineRef<S> &r, int q, S v)
+{
+    r(q) = v;
+}
+
+// x direction: lanes are rows, so reading one position of 64 rows touches
+// 64 cache lines.  XDescIO stages T-position tiles of the wave's 64 rows in
+// LDS (each row segment one 128-byte line, loaded coalesced), one tile
+// ahead in registers, in the order the transposed sweep reads: every read
+// lies below all earlier ones (the first window is read from its top).
+// Outputs collect in a second tile, written back coalesced when the sweep
+// moves below it; only positions in [a, b) are written, and the sweep emits
+// every one of them while its tile is current (the left end's last 12 in
+// the tile of position 0).
+template <typename S>
+struct XDescIO {
+    static constexpr int T = 128 / sizeof(S);
+    static constexpr int RP = 64 / T;  // rows per pass
+    const S *in;
+    S *out;
+    int pitch, n, m, row0, a, b, lane;
+    S (*tin)[T + 1];
+    S (*tout)[T + 1];
+    int tc;  // tile in LDS: [tc, tc + T)
+    int oc;  // output tile being filled: [oc, oc + T); -1 none
+    S pre[T];  // the tile below, in flight
+    __device__ __forceinline__ void fetch(S (&dst)[T], int t)
+    {
+        // clamped indices, every load issued: values outside the grid land
+        // in slots that are never read
+        const int c = lane % T, rs = lane / T;
+        int gi = t + c;
+        gi = gi < 0 ? 0 : (gi < n ? gi : n - 1);
+#pragma unroll
+        for (int rr = 0; rr < T; ++rr) {
+            int gj = row0 + rr * RP + rs;
+            gj = gj < m ? gj : m - 1;
+            dst[rr] = in[(size_t)gj * pitch + gi];
+        }
+    }
+    __device__ __forceinline__ void commit()
+    {
+        const int c = lane % T, rs = lane / T;
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < T; ++rr) tin[rr * RP + rs][c] = pre[rr];
+        __syncthreads();
+    }
+    __device__ __forceinline__ void start(int p_top)  // the first read is position p_top
+    {
+        tc = p_top & ~(T - 1);
+        fetch(pre, tc);
+        commit();
+        fetch(pre, tc - T);
+        oc = -1;
+    }
+    __device__ __forceinline__ S load(int p)
+    {
+        if (p < tc) {  // uniform: all lanes read the same positions
+            commit();
+            tc -= T;
+            fetch(pre, tc - T);
+        }
+        return tin[lane][p - tc];
+    }
+    __device__ __forceinline__ void flush()
+    {
+        if (oc < 0) return;
+        const int c = lane % T, rs = lane / T;
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < T; ++rr) {
+            const int gj = row0 + rr * RP + rs, gi = oc + c;
+            if (gj < m && gi >= a && gi < b) out[(size_t)gj * pitch + gi] = tout[rr * RP + rs][c];
+        }
+        __syncthreads();
+    }
+    __device__ __forceinline__ void emit(int e, S v)
+    {
+        const int t = e & ~(T - 1);
+        if (t != oc) {
+            flush();
+            oc = t;
+        }
+        tout[lane][e - oc] = v;
+    }
+};
+
+template <typename S>
+__device__ __forceinline__ S line_rd(XDescIO<S> &io, int q)
+{
+    return io.load(q);
+}
+template <typename S>
+__device__ __forceinline__ void line_wr(XDescIO<S> &io, int q, S v)
+{
+    io.emit(q, v);
+}
+
+template <typename S, bool YDIR, typename IN, typename OUT>
+__host__ __device__ __forceinline__ void adj_filter_line(IN &In, OUT &Out, int n, S sigma, int a, int b, int W)
 {
     const bool exact_right = b + 6 + W >= n - 7;
     int q = exact_right ? n - 7 : b + 6 + W;
     const int qs = a - 6 > 6 ? a - 6 : 6;
     S w[13];
-    // window = positions q-6 .. q+6 (slot k = position q-6+k)
+    // window = positions q-6 .. q+6 (slot k = position q-6+k), read from the top
 #pragma unroll
-    for (int k = 0; k < 13; ++k) w[k] = In(q - 6 + k);
+    for (int k = 12; k >= 0; --k) w[k] = line_rd(In, q - 6 + k);
     if (exact_right) {
         // R^T: ops n-7 .. n-2 ascending; slot of position p is p-(n-13)
 #pragma unroll
@@ -144,10 +248,10 @@ __host__ __device__ __forceinline__ void adj_filter_line(const LineRef<const S> 
             const int sl = (12 - u + 13) % 13;  // position q-u+6, final now
             const int e = q - u + 6;
             if constexpr (M == 1) {
-                if (e >= a && e < b) Out(e) = w[sl];
+                if (e >= a && e < b) line_wr(Out, e, w[sl]);
             }
-            if constexpr (M == 2) Out(e) = w[sl];
-            w[sl] = In(q - u - 7);
+            if constexpr (M == 2) line_wr(Out, e, w[sl]);
+            w[sl] = line_rd(In, q - u - 7);
         }
         q -= 13;
     };
@@ -167,12 +271,12 @@ __host__ __device__ __forceinline__ void adj_filter_line(const LineRef<const S> 
                     w[sl] = w[sl] - S(filt_d(s + 6)) * t;
                 }
                 const int sl = (12 - u + 13) % 13;  // position q+6, final now
-                if (q + 6 >= a && q + 6 < b) Out(q + 6) = w[sl];
+                if (q + 6 >= a && q + 6 < b) line_wr(Out, q + 6, w[sl]);
                 if (q == qs) {
                     rot = u;
                     done = true;
                 } else {
-                    w[sl] = In(q - 7);
+                    w[sl] = line_rd(In, q - 7);
                     --q;
                 }
             }
@@ -198,30 +302,47 @@ __host__ __device__ __forceinline__ void adj_filter_line(const LineRef<const S> 
         }
 #pragma unroll
         for (int k = 0; k < 12; ++k)
-            if (k >= a && k < b) Out(k) = v[k];
+            if (k >= a && k < b) line_wr(Out, k, v[k]);
     }
 }
 
 template <typename S, bool YDIR>
 __global__ __launch_bounds__(64) void adj_filter_kernel(AdjFilterArgs<S> F)
 {
-    const int line = blockIdx.x * 64 + threadIdx.x;
     const int k = blockIdx.z;
-    if (line >= F.m || k >= F.nk) return;
+    if (k >= F.nk) return;
     int a, b;
     if (!chunk_bounds(blockIdx.y, F.L, F.n, a, b)) return;
-    LineRef<const S> In;
-    LineRef<S> Out;
-    if (YDIR) {  // line = column i, positions along j
+    if constexpr (YDIR) {  // line = column i, positions along j (coalesced across lanes)
+        const int line = blockIdx.x * 64 + threadIdx.x;
+        if (line >= F.m) return;
+        LineRef<const S> In;
+        LineRef<S> Out;
         In.p = F.in[k] + line;
         Out.p = F.out[k] + line;
         In.st = Out.st = (size_t)F.pitch;
-    } else {  // line = row j, positions along i
-        In.p = F.in[k] + (size_t)line * F.pitch;
-        Out.p = F.out[k] + (size_t)line * F.pitch;
-        In.st = Out.st = 1;
+        adj_filter_line<S, true>(In, Out, F.n, S(F.sigma), a, b, F.W);
+    } else {  // line = row j: every lane takes part in the tile loads (rows past the grid clamp)
+        constexpr int T = XDescIO<S>::T;
+        __shared__ S tin[64][T + 1];
+        __shared__ S tout[64][T + 1];
+        XDescIO<S> io;
+        io.in = F.in[k];
+        io.out = F.out[k];
+        io.pitch = F.pitch;
+        io.n = F.n;
+        io.m = F.m;
+        io.row0 = blockIdx.x * 64;
+        io.a = a;
+        io.b = b;
+        io.lane = threadIdx.x;
+        io.tin = tin;
+        io.tout = tout;
+        const bool exact_right = b + 6 + F.W >= F.n - 7;
+        io.start((exact_right ? F.n - 7 : b + 6 + F.W) + 6);
+        adj_filter_line<S, false>(io, io, F.n, S(F.sigma), a, b, F.W);
+        io.flush();
     }
-    adj_filter_line<S, YDIR>(In, Out, F.n, S(F.sigma), a, b, F.W);
 }
 
 // ---------------------------------------------------------------------------

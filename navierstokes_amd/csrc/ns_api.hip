@@ -581,13 +581,13 @@ int autotune_stage_t(ns_ctx *c)
     // look-back start is exact to 1e-19 relative, which can still flip a
     // rounding now and then).
     S *S2 = static_cast<S *>(c->S[2]);
-    const int Ls[4] = {128, 256, 512, 1024};
+    const int Ls[5] = {64, 128, 256, 512, 1024};
     auto tune_L = [&](bool xdir, int &L, double &best_ms) -> int {
         const int n = xdir ? P.imax : P.jmax;
         if (n < 272) return NS_OK;
         int bestL = L;
         best_ms = 1e300;
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < 5; ++t) {
             if (Ls[t] >= n && t > 0) break;
             L = Ls[t];
             auto sweep = [&]() -> hipError_t {
