@@ -162,3 +162,21 @@ def test_batch_rejects_mixed_nk():
     cases = [config.acoustic_pulse_case(64, D=8), config.random_case(64, 64, nk=5, D=8, seed=1)]
     with pytest.raises(NSError):
         BatchSolver(cases)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_batch_metric_grids_fp64_bitwise(variant):
+    """Batches of cases on non-unit grids (periodic mixing layers: x spacing
+    0.05, tanh-stretched y, viscous): the marching variants run the general
+    per-point code with 1/h factors on every block (no fast zone); every
+    variant gives the single contexts' FP64 bits."""
+    from navierstokes_amd.solver import BatchSolver
+    cases = [config.mixing_layer_case(160, 96, D=10, seed=5), config.mixing_layer_case(192, 128, D=12, seed=6)]
+    print(f"[case] metric-grid batch {[(c.imax, c.jmax) for c in cases]} FP64 variant {variant}", flush=True)
+    ref = _singles(cases, FP64, 4)
+    b = BatchSolver(cases, precision=FP64)
+    b.set_variant(variant)
+    b.run(0, 4)
+    for i, c in enumerate(cases):
+        _same(b.get_state(i), ref[i], c, FP64, f"variant {variant} case {i}")
+    b.close()
